@@ -1,0 +1,206 @@
+"""Benchmark: Siddon projector + backprojector pair on one MI355X.
+
+Workload (configuration C2): 256^3 fp32 cube over [-1.5, 1.5]^3, 128 conic
+views of 512 x 512 pixels from 214 R_sun, longitudes 2 pi k / 128, latitudes
+U(-7, 7) deg, a seeded 40-blob phantom, images = P(phantom) + 1% noise.  A
+"step" is one application of the normal operator's projector pair, as a CG
+iteration makes it: zero the image stack, project the cube into it, zero the
+cube buffer, backproject the images (siddon_lo matvec + rmatvec,
+lo_wrapper.py:23-34).  Inputs are resident in HBM before timing starts.
+
+value = Siddon steps per second (voxel segments traversed by the pair / time).
+roofline: the dominant kernel's algorithmic bytes per launch over its average
+launch time (HIP events on the launch stream), against 8 TB/s HBM.
+  forward  bytes = S*b + R_t*b + V*18*b   (voxel gather per step, pixel write per ray)
+  backward bytes = 2*S*b + R_t*b + V*18*b (voxel read-modify-write per step, pixel read per ray)
+cpu_baseline: the oracle (C/OpenMP restatement of the reference) on a bounded
+sample of the same workload's views, on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--dtype", default="float32", choices=["float32", "float64"])
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--views", type=int, default=128)
+    ap.add_argument("--pix", type=int, default=512)
+    ap.add_argument("--tile-j", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cg-iters", type=int, default=0, help="also time this many CG iterations")
+    return ap.parse_args()
+
+
+def cpu_baseline(ch, hs, cube_np, data_np, target_s):
+    """Oracle fwd+bwd on a bounded subset of the views, all host threads."""
+    from oracle import oracle as O
+    from tomograpy_amd.headers import map_record, view_records
+    recs = view_records(hs)
+    mrec = map_record(ch)
+    threads = O.num_threads()
+    nv = len(hs)
+    # calibrate on 2 views, then size the sample to about target_s of CPU time
+    k = 2
+    t0 = time.perf_counter()
+    d = np.zeros_like(data_np)
+    c = np.zeros_like(cube_np)
+    s1 = O.siddon(O.PJ, recs, mrec, cube_np, d, t0=0, t1=k, nthread=threads)
+    s2 = O.siddon(O.BPJ, recs, mrec, c, data_np, t0=0, t1=k, nthread=threads)
+    dt = time.perf_counter() - t0
+    per_view = dt / k
+    k2 = int(max(k, min(nv, target_s / max(per_view, 1e-6))))
+    # OpenMP splits views over threads: use a multiple of the thread count when possible
+    if k2 >= threads:
+        k2 = (k2 // threads) * threads
+    t0 = time.perf_counter()
+    s1 = O.siddon(O.PJ, recs, mrec, cube_np, d, t0=0, t1=k2, nthread=threads)
+    s2 = O.siddon(O.BPJ, recs, mrec, c, data_np, t0=0, t1=k2, nthread=threads)
+    dt = time.perf_counter() - t0
+    return {"value": (s1 + s2) / dt, "unit": "Siddon steps/s", "cores": threads,
+            "kind": "port",
+            "sample": "oracle pj+bpj on views 0..%d of the workload (%d of %d), %.1f s" %
+                      (k2 - 1, k2, nv, dt)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only; --gpus %d is not supported (the path would shard "
+              "by views with one all-reduce of the backprojected cube)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import tomograpy_amd as T
+    from tomograpy_amd import workloads as W
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    from tomograpy_amd import cg as CG
+
+    torch.cuda.set_device(0)
+    dtype = np.dtype(args.dtype)
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    b = dtype.itemsize
+    ch, hs, rng = W.config2_geometry(dtype, n=args.n, n_views=args.views, n_pix=args.pix)
+    phantom = W.blob_phantom(ch, 40, rng, dtype=dtype)
+    P = siddon_lo(hs, ch, dtype=dtype, tile_j=args.tile_j)
+    x_true = torch.from_numpy(phantom).cuda()
+    y = P.matvec(x_true).clone()
+    noise = torch.from_numpy(rng.normal(size=tuple(y.shape)).astype(dtype)).cuda()
+    y += 0.01 * float(y.max()) * noise
+    S = P.count_steps()
+    R_t = P._rays_traced
+    V = len(hs)
+    torch.cuda.synchronize()
+
+    x = x_true.clone()
+    stream = torch.cuda.current_stream()
+
+    def step():
+        P.xout.zero_()
+        P.project_into(x, P.xout)
+        P.xin.zero_()
+        P.backproject_into(y, P.xin)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+
+    # per-kernel average launch time, events on the launch stream
+    fe = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    be = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    for k in range(args.steps):
+        P.xout.zero_()
+        fe[2 * k].record(stream)
+        P.project_into(x, P.xout)
+        fe[2 * k + 1].record(stream)
+        P.xin.zero_()
+        be[2 * k].record(stream)
+        P.backproject_into(y, P.xin)
+        be[2 * k + 1].record(stream)
+    torch.cuda.synchronize()
+    t_fwd = np.mean([fe[2 * k].elapsed_time(fe[2 * k + 1]) for k in range(args.steps)]) * 1e-3
+    t_bwd = np.mean([be[2 * k].elapsed_time(be[2 * k + 1]) for k in range(args.steps)]) * 1e-3
+    T.check_status()
+    B_fwd = S * b + R_t * b + V * 18 * b
+    B_bwd = 2 * S * b + R_t * b + V * 18 * b
+    if t_bwd >= t_fwd:
+        dom, B_dom, t_dom = "siddon_backproject", B_bwd, t_bwd
+    else:
+        dom, B_dom, t_dom = "siddon_project", B_fwd, t_fwd
+    achieved = B_dom / t_dom / 1e9
+
+    extra = {}
+    if args.cg_iters:
+        A = CG.NormalOperator(P, lam=1e-2, identity=True)
+        bvec = torch.zeros(P.shapein, dtype=tdt, device="cuda")
+        P.backproject_into(y, bvec)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        _, info = CG.cg(A, bvec, maxiter=args.cg_iters)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        extra["cg"] = {"iterations": info["iterations"], "ms_per_iteration":
+                       e0.elapsed_time(e1) / info["iterations"], "residual": info["residual"]}
+
+    cpu = None
+    if not args.no_cpu:
+        data_np = np.zeros((args.pix, args.pix, V), dtype=dtype)
+        y_np = y.cpu().numpy()
+        cpu = cpu_baseline(ch, hs, phantom, np.ascontiguousarray(y_np), args.cpu_seconds)
+
+    value = 2 * S / (ms_step * 1e-3)
+    out = {
+        "metric": "Siddon steps/s (projector+backprojector pair)",
+        "value": value,
+        "unit": "Siddon steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if dtype == np.float32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": "C2 pair: %d^3 cube, %d views of %d^2, D=214, LAT U(-7,7) deg"
+                               % (args.n, V, args.pix),
+                   "siddon_steps_per_projection": S, "rays_traced": R_t,
+                   "ms_forward": t_fwd * 1e3, "ms_backward": t_bwd * 1e3},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": B_dom},
+        "cpu_baseline": cpu,
+        "wall_s_timed": t_wall,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * tomograpy_amd -- C ABI of the MI355X (gfx950) Siddon conic projector /
+ * backprojector library.
+ *
+ * Every entry point is a plain C function over POD structs, raw device
+ * pointers and sizes.  The caller owns every buffer (the Python layer uses the
+ * torch allocator); the library allocates nothing on the device except inside
+ * tg_batch_* (documented there).  Calls are asynchronous on the caller's HIP
+ * stream and return 0 on success or a negative TG_E* code; tg_last_error()
+ * returns a thread-local message for the last failing call on this thread.
+ *
+ * Reference interfaces replaced (tomograpy/C_siddon.c.template = "C:"):
+ *   16 generated modules _C_siddon_<ctype>_<obstacle>_<pj> (parse_templates.py:11-26,
+ *   setup.py:13-22), each exporting the 7 functions of C:163-172.  Here the
+ *   ctype, obstacle and pj kind are fields of tg_call instead of module names.
+ */
+#ifndef TOMOGRAPY_AMD_H
+#define TOMOGRAPY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element types (parse_templates.py:11 "float" / "double"). */
+#define TG_F32 0
+#define TG_F64 1
+
+/* Update kinds (C:915-944: pj, bpj, pjt, bpjt). */
+#define TG_PJ 0
+#define TG_BPJ 1
+#define TG_PJT 2
+#define TG_BPJT 3
+
+/* tg_call.flags */
+#define TG_OBSTACLE_SUN 1u   /* stop a ray at the unit sphere (C:956-961) */
+#define TG_DETERMINISTIC 2u  /* bpj/bpjt: sum each voxel in the reference's serial
+                                order (view, row, column), bit-exact to nthread=1 */
+
+/* Error codes */
+#define TG_OK 0
+#define TG_EINVAL -1     /* bad argument (shape, stride, dtype, kind) */
+#define TG_EHIP -2       /* a HIP runtime call failed */
+#define TG_EWORKSPACE -3 /* deterministic mode: workspace missing or too small */
+#define TG_EGEOMETRY -4  /* a ray's traversal state is NaN (the reference would spin forever) */
+
+/* Per-view header record: 18 doubles, the keys C:1077-1101 reads, in order. */
+#define TG_HDR 18
+enum {
+  TG_H_CRPIX1 = 0, TG_H_CRPIX2, TG_H_CDELT1, TG_H_CDELT2, TG_H_CRVAL1, TG_H_CRVAL2,
+  TG_H_R11, TG_H_R12, TG_H_R13, TG_H_R21, TG_H_R22, TG_H_R23, TG_H_R31, TG_H_R32, TG_H_R33,
+  TG_H_M1, TG_H_M2, TG_H_M3
+};
+
+/* Map cube: the keys C:1058-1075 reads (CDELT, PSHAPE, MMIN, MMAX per axis),
+ * the cube shape and its element strides.  stride[3] is the time-bin stride of a
+ * 4-D map (C:32-35); nt is the number of time bins (1 for a 3-D map). */
+typedef struct tg_map {
+  double cdelt[3], pshape[3], mmin[3], mmax[3];
+  int32_t n[3];
+  int32_t nt;
+  int64_t stride[4];
+} tg_map;
+
+/* One projector / backprojector invocation (C:460-498). */
+typedef struct tg_call {
+  int32_t dtype;  /* TG_F32 | TG_F64 */
+  int32_t kind;   /* TG_PJ .. TG_BPJT */
+  uint32_t flags; /* TG_OBSTACLE_SUN | TG_DETERMINISTIC */
+  int32_t tile_j; /* pixels per block row (power of two, 1..256); 0 = default */
+  tg_map map;
+  /* image stack, shape (n1, n2, nv) as in the reference (images on the last axis) */
+  int32_t n1, n2, nv, pad0;
+  int64_t data_stride[3];
+  const void *views; /* device: packed view descriptors, see tg_views_pack */
+  void *cube;        /* device: map cube (in for pj, accumulated for bpj) */
+  void *data;        /* device: images (accumulated for pj, in for bpj) */
+  /* optional operands (NULL = absent) */
+  const uint8_t *pixmask;   /* device: nonzero = skip this pixel's ray (C:520) */
+  int64_t mask_stride[3];
+  const int32_t *tindex;    /* device: time bin per view for pjt/bpjt; NULL = view index (C:935) */
+  const void *pix_weight;   /* device: per-pixel weight, data strides (diag O of O.P.M) */
+  const void *vox_weight;   /* device: per-voxel weight, vox_stride (diag M of O.P.M) */
+  const uint8_t *voxmask;   /* device: nonzero = voxel excluded (P.Mo^T) */
+  int64_t vox_stride[3];
+  int32_t *status;          /* device int32 or NULL: set to TG_EGEOMETRY if a ray stalls */
+  void *workspace;          /* device scratch for TG_DETERMINISTIC */
+  size_t workspace_bytes;
+} tg_call;
+
+/* Size in bytes of the packed view descriptors for nv views of n1 x n2 pixels. */
+size_t tg_views_bytes(int32_t dtype, int32_t nv, int32_t n1, int32_t n2);
+
+/* Packs nv header records (nv * TG_HDR doubles) into host memory `out`
+ * (tg_views_bytes bytes): the values rounded to the element type as
+ * C:1051-1056 does, plus per-view tables of cos/sin of every row angle gamma
+ * and column angle lambda computed with libm in double precision, as
+ * C:1013-1018 and C:1043-1046 compute them.  The caller copies `out` to the
+ * device and passes it as tg_call.views.  Replaces the header marshalling of
+ * C:477-487 (PyDict_AsImageHeader, C:1077-1101). */
+int tg_views_pack(int32_t dtype, int32_t nv, int32_t n1, int32_t n2, const double *hdr, void *out,
+                  size_t out_bytes);
+
+/* All views: replaces conic_full_projector (C:182-205, C:460-498) for the
+ * kind/dtype/obstacle selected in *c (siddon.py:85-219 select the module). */
+int tg_conic_full_projector(const tg_call *c, void *stream);
+
+/* One view t: replaces conic_image_projector (C:207-248, C:500-528). */
+int tg_conic_image_projector(const tg_call *c, int32_t t, void *stream);
+
+/* One ray (pixel i, j of view t): replaces ray_projector (C:250-307).  A pixel
+ * whose mask is set is skipped (C:284-286 tests mask == 1; here nonzero). */
+int tg_ray_projector(const tg_call *c, int32_t i, int32_t j, int32_t t, void *stream);
+
+/* Unit vectors u[i, j, t, 0:3] (C-contiguous, element type of c->dtype) of
+ * views [t0, t1); pixels whose data value is NaN are left untouched (C:638).
+ * Replaces full_unit_vector (C:310-330, 582-616) with t0 = 0, t1 = nv and
+ * image_unit_vector (C:332-365, 618-648) with t1 = t0 + 1. */
+int tg_unit_vector(const tg_call *c, int32_t t0, int32_t t1, void *u, void *stream);
+
+/* Intersection parameters a1/an (C:723-735) from u, views [t0, t1), same
+ * layout as u.  Replaces full_intersection_parameters (C:368-399, 650-692)
+ * and image_intersection_parameters (C:402-455, 694-720). */
+int tg_intersection_parameters(const tg_call *c, int32_t t0, int32_t t1, const void *u, void *a1,
+                               void *an, void *stream);
+
+/* Siddon step count (the number of voxel() calls C:575-578 would make) per ray,
+ * written to per_ray[t][i][j] (may be NULL), and added to *total (device
+ * uint64, may be NULL).  No reference counterpart: used for throughput units. */
+int tg_count_steps(const tg_call *c, uint32_t *per_ray, unsigned long long *total, void *stream);
+
+/* Minimum workspace for TG_DETERMINISTIC backprojection of *c.  Larger
+ * workspaces let more views be processed per pass. */
+size_t tg_det_workspace_min_bytes(const tg_call *c);
+
+/* Elementwise x[k] *= w[k] over a strided 3-D (or 4-D with nt bins) cube: the
+ * voxel-weight half of the Thomson operator's adjoint (models.py:223-235). */
+int tg_scale_cube(const tg_call *c, void *stream);
+
+/* ---- device-resident conjugate gradient (lo.acg consumer, srt.py:225) ---- */
+
+/* out += lam * ((identity ? x : 0) + sum_k D_k^T D_k x) on a C-contiguous
+ * array of shape[0..3] (unused trailing dims = 1), with D_k the forward
+ * difference along axis k in the numpy.diff convention (output shorter by
+ * one), k selected by bit k of axes_mask.  Entries whose voxmask byte is
+ * nonzero are left untouched (voxmask may be NULL).  Replaces the smoothness
+ * priors lo.diff of models.py:85-94 inside the CG normal equations. */
+int tg_regularize(int32_t dtype, const int32_t shape[4], uint32_t axes_mask, double lam,
+                  int32_t identity, const void *x, void *out, const uint8_t *voxmask,
+                  void *stream);
+
+/* Fused CG vector kernels over contiguous length-n vectors (the lo.acg
+ * consumer of the projector pair, srt.py:225).  `scal` is a device array of
+ * TG_CG_SCAL doubles:
+ *   scal[0] = r.r, scal[1] = p.q, scal[2] = new r.r, scal[3] = alpha,
+ *   scal[4] = beta, scal[5..15] free for the caller, scal[16..] partial sums.
+ * Reductions run in a fixed order, so results are reproducible.
+ * tg_cg_dot:       scal[slot] = sum a*b in double (slot 0..15).
+ * tg_cg_update:    alpha = scal[0]/scal[1]; x += alpha p; r -= alpha q;
+ *                  scal[2] = r.r.
+ * tg_cg_direction: beta = scal[2]/scal[0]; p = r + beta p; scal[0] = scal[2]. */
+#define TG_CG_SCAL (16 + 1024)
+int tg_cg_dot(int32_t dtype, int64_t n, const void *a, const void *b, double *scal, int32_t slot,
+              void *stream);
+int tg_cg_update(int32_t dtype, int64_t n, void *x, void *r, const void *p, const void *q,
+                 double *scal, void *stream);
+int tg_cg_direction(int32_t dtype, int64_t n, const void *r, void *p, double *scal, void *stream);
+
+/* Thread-local message describing the last error on this thread. */
+const char *tg_last_error(void);
+
+/* Library version string. */
+const char *tg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOMOGRAPY_AMD_H */

@@ -1,0 +1,276 @@
+"""GPU parity: the HIP library against the CPU oracle on identical inputs.
+
+Tolerances (written per test):
+  * forward projection (pj/pjt): bit-exact -- each pixel's sum is sequential
+    in the reference (C_siddon.c.template:915-920) and on the GPU;
+  * backprojection, atomic mode: relative L2 <= 1e-5 (fp32), 1e-12 (fp64)
+    against the oracle's single-thread order;
+  * backprojection, deterministic mode: bit-exact to the single-thread order;
+  * step counts, unit vectors, intersection parameters: exact.
+"""
+import numpy as np
+import pytest
+import torch
+
+import tomograpy_amd as T
+from tomograpy_amd import workloads as W
+from tomograpy_amd.headers import map_record, view_records
+from tests import cases as C
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den else 1.0)
+
+
+def c1_problem(dtype, n_views=32):
+    ch, cube, hs = W.config1(dtype)
+    hs = hs[:n_views]
+    data = T.InfoArray(np.zeros((128, 128, len(hs)), dtype=dtype), header=hs)
+    return ch, T.InfoArray(cube, header=dict(ch)), data
+
+
+def to_dev(x):
+    t = torch.from_numpy(np.ascontiguousarray(np.asarray(x))).cuda()
+    t.header = x.header
+    return t
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("obstacle", [None, "sun"])
+def test_forward_c1_bit_exact(oracle, dtype, obstacle):
+    print("forward C1", np.dtype(dtype).name, obstacle, flush=True)
+    ch, cube, data = c1_problem(dtype)
+    rng = np.random.default_rng(5)
+    data[:] = rng.uniform(-1, 1, data.shape).astype(dtype)  # accumulate into nonzero data
+    ref = data.copy()
+    oracle.siddon(oracle.PJ, view_records(data.header), map_record(ch), cube, ref,
+                  sun=obstacle == "sun")
+    out = data.copy()
+    T.projector(out, cube, obstacle=obstacle)
+    assert np.array_equal(np.asarray(out), np.asarray(ref))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("obstacle", [None, "sun"])
+def test_backward_c1_atomic_within_tolerance(oracle, dtype, obstacle):
+    print("backward atomic C1", np.dtype(dtype).name, obstacle, flush=True)
+    ch, cube, data = c1_problem(dtype)
+    rng = np.random.default_rng(6)
+    data[:] = rng.uniform(0, 1, data.shape).astype(dtype)
+    ref = T.InfoArray(np.zeros(cube.shape, dtype), header=dict(ch))
+    oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref, data,
+                  sun=obstacle == "sun", nthread=1)
+    out = T.InfoArray(np.zeros(cube.shape, dtype), header=dict(ch))
+    T.backprojector(data, out, obstacle=obstacle)
+    tol = 1e-5 if dtype == np.float32 else 1e-12
+    assert rel_l2(out, ref) <= tol
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("obstacle", [None, "sun"])
+def test_backward_c1_deterministic_bit_exact(oracle, dtype, obstacle):
+    print("backward deterministic C1", np.dtype(dtype).name, obstacle, flush=True)
+    ch, cube, data = c1_problem(dtype)
+    rng = np.random.default_rng(7)
+    data[:] = rng.uniform(-1, 1, data.shape).astype(dtype)
+    init = rng.uniform(0, 1, cube.shape).astype(dtype)  # accumulate into nonzero cube
+    ref = T.InfoArray(init.copy(), header=dict(ch))
+    oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref, data,
+                  sun=obstacle == "sun", nthread=1)
+    out = T.InfoArray(init.copy(), header=dict(ch))
+    T.backprojector(data, out, obstacle=obstacle, deterministic=True)
+    assert np.array_equal(np.asarray(out), np.asarray(ref))
+
+
+def test_deterministic_small_workspace_chunks_views(oracle):
+    """A workspace that holds only a few views' records still gives the exact order."""
+    import ctypes
+    from tomograpy_amd import _native as N, siddon as S
+    ch, x, data = C.small_case(n=16, npix=32, nv=9, dtype=np.float32)
+    rng = np.random.default_rng(8)
+    y = T.InfoArray(rng.normal(size=data.shape).astype(np.float32), header=data.header)
+    ref = np.zeros(x.shape, np.float32)
+    oracle.siddon(oracle.BPJ, view_records(y.header), map_record(ch), ref, y, nthread=1)
+    yd = torch.from_numpy(np.asarray(y)).cuda()
+    out = torch.zeros(x.shape, dtype=torch.float32, device="cuda")
+    views = S.views_tensor(view_records(y.header), N.TG_F32, 32, 32, out.device)
+    call = S.make_call(N.TG_BPJ, N.TG_F32, yd.shape, yd.stride(), out.shape, out.stride(),
+                       map_record(ch), views, out.data_ptr(), yd.data_ptr(), flags=N.TG_DETERMINISTIC)
+    need = N.lib().tg_det_workspace_min_bytes(ctypes.byref(call))
+    ws = torch.empty(int(need), dtype=torch.uint8, device="cuda")
+    call.workspace = ws.data_ptr()
+    call.workspace_bytes = int(need)
+    N.check(N.lib().tg_conic_full_projector(ctypes.byref(call), S.stream_handle()))
+    assert np.array_equal(out.cpu().numpy(), ref)
+    # too small a workspace is an error, not a wrong answer
+    call.workspace_bytes = 1024
+    assert N.lib().tg_conic_full_projector(ctypes.byref(call), S.stream_handle()) == N.TG_EWORKSPACE
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_step_counts_match_oracle(oracle, dtype):
+    ch, cube, data = c1_problem(dtype, n_views=8)
+    ref = oracle.ray_steps(map_record(ch), cube.shape, view_records(data.header), 128, 128)
+    total, per_ray = T.count_steps(data, cube, per_ray=True)
+    assert np.array_equal(per_ray.cpu().numpy().astype(np.uint32), ref)
+    assert total == int(ref.sum())
+    ref_sun = oracle.ray_steps(map_record(ch), cube.shape, view_records(data.header), 128, 128, sun=True)
+    assert T.count_steps(data, cube, obstacle="sun") == int(ref_sun.sum())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_unit_vectors_and_intersections_exact(oracle, dtype):
+    ch, cube, data = C.small_case(n=16, npix=32, nv=6, dtype=dtype)
+    data[3, 4, 1] = np.nan  # NaN pixels are skipped (C_siddon.c.template:638)
+    recs = view_records(data.header)
+    u_ref = oracle.full_unit_vector(recs, 32, 32, dtype, data=np.asarray(data))
+    u = T.C_full_unit_vector(data)
+    assert np.array_equal(u, u_ref)
+    assert np.all(u[3, 4, 1] == 0)
+    a1_ref, an_ref = oracle.full_intersection_parameters(map_record(ch), recs, u_ref, dtype)
+    a1, an = T.C_full_intersection_parameters(data, cube, u)
+    assert np.array_equal(a1, a1_ref) and np.array_equal(an, an_ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_mask_and_time_variants(oracle, dtype):
+    ch, cube, data = C.small_case(n=16, npix=32, nv=6, dtype=dtype)
+    rng = np.random.default_rng(11)
+    mask = rng.uniform(size=data.shape) < 0.3
+    recs = view_records(data.header)
+    # masked forward, bit-exact
+    ref = data.copy()
+    oracle.siddon(oracle.PJ, recs, map_record(ch), cube, ref, mask=mask)
+    out = data.copy()
+    T.projector(out, cube, mask=mask)
+    assert np.array_equal(np.asarray(out), np.asarray(ref))
+    # 4-D: image t reads bin t; and an explicit view -> bin map
+    cube4 = T.InfoArray(rng.uniform(size=(16, 16, 16, 6)).astype(dtype), header=dict(ch))
+    ref = data.copy()
+    oracle.siddon(oracle.PJT, recs, map_record(ch), np.asarray(cube4), ref)
+    out = data.copy()
+    T.projector4d(out, cube4)
+    assert np.array_equal(np.asarray(out), np.asarray(ref))
+    tindex = np.array([2, 0, 1, 2, 0, 1], np.int32)
+    cube3b = T.InfoArray(np.asarray(cube4)[..., :3].copy(), header=dict(ch))
+    ref = data.copy()
+    oracle.siddon(oracle.PJT, recs, map_record(ch), np.asarray(cube3b), ref, tindex=tindex)
+    out = data.copy()
+    T.projector4d(out, cube3b, tindex=tindex)
+    assert np.array_equal(np.asarray(out), np.asarray(ref))
+    # 4-D backprojection, deterministic: bit-exact
+    y = T.InfoArray(rng.normal(size=data.shape).astype(dtype), header=data.header)
+    ref4 = np.zeros((16, 16, 16, 3), dtype)
+    oracle.siddon(oracle.BPJT, recs, map_record(ch), ref4, y, tindex=tindex, nthread=1)
+    out4 = T.InfoArray(np.zeros((16, 16, 16, 3), dtype), header=dict(ch))
+    T.backprojector4d(y, out4, tindex=tindex, deterministic=True)
+    assert np.array_equal(np.asarray(out4), ref4)
+    # masked backprojection, atomic
+    refm = np.zeros(cube.shape, dtype)
+    oracle.siddon(oracle.BPJ, recs, map_record(ch), refm, y, mask=mask, nthread=1)
+    outm = T.InfoArray(np.zeros(cube.shape, dtype), header=dict(ch))
+    T.backprojector(y, outm, mask=mask)
+    assert rel_l2(outm, refm) <= (1e-5 if dtype == np.float32 else 1e-12)
+
+
+def test_image_and_ray_projectors(oracle):
+    ch, cube, data = C.small_case(n=16, npix=32, nv=6)
+    recs = view_records(data.header)
+    ref = data.copy()
+    oracle.siddon(oracle.PJ, recs, map_record(ch), cube, ref, t0=2, t1=3)
+    out = data.copy()
+    T.conic_image_projector(out, cube, 2)
+    assert np.array_equal(np.asarray(out), np.asarray(ref))
+    out = data.copy()
+    T.ray_projector(out, cube, (5, 7, 4))
+    iv, ln, _ = oracle.ray_trace(map_record(ch), cube.shape, recs[4], 5, 7)
+    v = 0.0
+    for k, l in zip(iv, ln):
+        v = v + l * np.asarray(cube)[tuple(k)]
+    assert np.asarray(out)[5, 7, 4] == v
+    assert np.count_nonzero(np.asarray(out)) == 1
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-5), (np.float64, 1e-12)])
+def test_adjoint_identity_gpu(dtype, tol):
+    ch, x, data = C.small_case(n=32, npix=48, nv=12, dtype=dtype, radius=6.0)
+    rng = np.random.default_rng(12)
+    xd = to_dev(x)
+    y = torch.from_numpy(rng.normal(size=data.shape).astype(dtype)).cuda()
+    y.header = data.header
+    px = torch.zeros(data.shape, dtype=xd.dtype, device="cuda")
+    px.header = data.header
+    T.projector(px, xd, obstacle="sun")
+    pty = torch.zeros(x.shape, dtype=xd.dtype, device="cuda")
+    pty.header = dict(ch)
+    T.backprojector(y, pty, obstacle="sun")
+    lhs = float((px.double() * y.double()).sum())
+    rhs = float((xd.double() * pty.double()).sum())
+    assert abs(lhs - rhs) <= tol * max(abs(lhs), abs(rhs))
+
+
+def test_reference_invariants_on_gpu():
+    """tests/test_siddon.py:88-115 on the GPU path."""
+    im_h = dict(C.image_headers(np.float64)[0])
+    im_h["CRPIX1"] = 1.0
+    im_h["CRPIX2"] = 1.0
+    data = C.stack(im_h, np.float64, n_images=1, fill=1.0)
+    for ch in C.object_headers(np.float64):
+        cube = T.InfoArray(np.zeros((ch["NAXIS1"],) * 3), header=dict(ch))
+        T.backprojector(data, cube)
+        assert np.sum(cube) == ch["PSHAPE1"]
+    data = C.stack(C.image_headers(np.float64)[0], np.float64, n_images=1, fill=1.0)
+    h1, h16 = C.object_headers(np.float64)
+    c16 = T.InfoArray(np.zeros((16, 16, 16)), header=dict(h16))
+    c1 = T.InfoArray(np.zeros((1, 1, 1)), header=dict(h1))
+    T.backprojector(data, c16, deterministic=True)
+    T.backprojector(data, c1, deterministic=True)
+    assert np.sum(c16) == c1[0, 0, 0]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_support_special_cases_gpu(dtype):
+    """tests/test_siddon.py:141-155: data != 0 exactly where the ray meets the cube."""
+    for ch in C.object_headers(dtype):
+        for im_h in C.image_headers(dtype):
+            data = C.stack(im_h, dtype, n_images=17)
+            cube = T.InfoArray(np.ones((ch["NAXIS1"],) * 3, dtype=dtype), header=dict(ch))
+            T.projector(data, cube)
+            u = T.C_full_unit_vector(data)
+            a1, an = T.C_full_intersection_parameters(data, cube, u)
+            flag = np.minimum(a1, an).max(-1) < np.maximum(a1, an).min(-1)
+            assert np.array_equal(np.asarray(data) != 0, flag)
+
+
+def test_c2_shape_views_bit_exact(oracle):
+    """Full C2 shapes (256^3, 512^2 images) on three views: forward bit-exact."""
+    ch, hs, rng = W.config2_geometry(np.float32)
+    sel = [0, 37, 101]
+    hs = [hs[k] for k in sel]
+    cube = T.InfoArray(W.blob_phantom(ch, 40, rng), header=dict(ch))
+    data = T.InfoArray(np.zeros((512, 512, 3), np.float32), header=hs)
+    ref = data.copy()
+    oracle.siddon(oracle.PJ, view_records(hs), map_record(ch), cube, ref)
+    out = data.copy()
+    T.projector(out, cube)
+    assert np.array_equal(np.asarray(out), np.asarray(ref))
+
+
+def test_torch_inputs_accumulate_in_place():
+    ch, cube, data = C.small_case(n=16, npix=32, nv=4, dtype=np.float32)
+    x = to_dev(cube)
+    y = torch.full(data.shape, 2.0, dtype=torch.float32, device="cuda")
+    y.header = data.header
+    y0 = y.clone()
+    T.projector(y, x)
+    y1 = y.clone()
+    T.projector(y, x)
+    torch.cuda.synchronize()
+    T.check_status()
+    # second call adds the same projection again (in-place accumulation)
+    assert torch.allclose(y - y1, y1 - y0, rtol=1e-5, atol=1e-5)

@@ -1,0 +1,115 @@
+"""CPU checks of the C-ABI library: it loads, exports every declared symbol,
+packs view descriptors exactly, and rejects bad arguments -- no GPU needed."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import tomograpy_amd as T
+from tomograpy_amd import _native as N
+from tomograpy_amd.headers import view_records
+from tests import cases as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", fn)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names += re.findall(r"\b(tg_[a-z0-9_]+)\s*\(", src)
+    return sorted(set(names))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = N.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    bound = {s[0] for s in N.SIGNATURES}
+    assert set(names) <= bound, set(names) - bound
+    assert b"gfx950" in lib.tg_version()
+
+
+def test_struct_layout_matches_header():
+    # sizes of the POD structs as the C compiler lays them out
+    assert ctypes.sizeof(N.TgMap) == 12 * 8 + 4 * 4 + 4 * 8
+    assert ctypes.sizeof(N.TgCall) % 8 == 0
+    assert N.TgCall.map.offset == 16
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_views_pack_matches_oracle_unit_vectors(oracle, dtype):
+    """The packed cos/sin tables + CT-rounded headers give the oracle's unit vectors
+    when the kernel's arithmetic is replayed in numpy in the same order."""
+    ch, cube, data = C.small_case(npix=13, nv=5, dtype=dtype)
+    recs = view_records(data.header)
+    code = N.TG_F32 if dtype == np.float32 else N.TG_F64
+    n1, n2, nv = data.shape
+    nbytes = N.lib().tg_views_bytes(code, nv, n1, n2)
+    buf = np.zeros(nbytes, dtype=np.uint8)
+    N.check(N.lib().tg_views_pack(code, nv, n1, n2, recs.ctypes.data, buf.ctypes.data, nbytes))
+    hdr_bytes = (np.dtype(dtype).itemsize * 18 * nv + 255) // 256 * 256
+    hdr = buf[: 18 * nv * np.dtype(dtype).itemsize].view(dtype).reshape(nv, 18)
+    np.testing.assert_array_equal(hdr, recs.astype(dtype))
+    trig = buf[hdr_bytes:].view(np.float64).reshape(nv, n1 + n2, 2)
+    u_ref = oracle.full_unit_vector(recs, n1, n2, dtype)
+    for t in range(nv):
+        R = hdr[t, 6:15].reshape(3, 3)
+        cg, sg = trig[t, :n1, 0][:, None], trig[t, :n1, 1][:, None]
+        cl, sl = trig[t, n1:, 0][None, :], trig[t, n1:, 1][None, :]
+        u2 = [(cl * cg).astype(dtype), (cl * sg).astype(dtype), np.broadcast_to(sl, (n1, n2)).astype(dtype)]
+        for k in range(3):
+            uk = (R[k, 0] * u2[0] + R[k, 1] * u2[1]) + R[k, 2] * u2[2]
+            np.testing.assert_array_equal(uk, u_ref[:, :, t, k])
+
+
+def test_views_pack_rejects_small_buffer():
+    recs = np.zeros((2, 18))
+    buf = np.zeros(16, dtype=np.uint8)
+    rc = N.lib().tg_views_pack(N.TG_F32, 2, 4, 4, recs.ctypes.data, buf.ctypes.data, 16)
+    assert rc == N.TG_EINVAL
+    assert b"too small" in N.lib().tg_last_error()
+    with pytest.raises(N.NativeError):
+        N.check(rc)
+
+
+def test_bad_dtype_and_headers_raise_before_device():
+    ch, cube, data = C.small_case(dtype=np.float64)
+    d32 = T.InfoArray(np.zeros(data.shape, np.float32), header=data.header)
+    with pytest.raises(ValueError, match="same data-type"):
+        T.projector(d32, cube)
+    bad = T.InfoArray(np.zeros(data.shape), header=tuple(data.header))
+    with pytest.raises(ValueError, match="should be a list"):
+        T.projector(bad, cube)
+    di = T.InfoArray(np.zeros(data.shape, np.int32), header=data.header)
+    ci = T.InfoArray(np.zeros(cube.shape, np.int32), header=cube.header)
+    with pytest.raises(KeyError):
+        T.projector(di, ci)
+    with pytest.raises(KeyError):
+        T.projector(data, cube, obstacle="moon")
+
+
+def test_call_validation_errors_without_gpu():
+    """make_args rejects bad shapes before any launch."""
+    call = N.TgCall()
+    call.dtype = N.TG_F32
+    call.kind = N.TG_PJ
+    call.nv = 1
+    call.n1 = call.n2 = 4
+    rc = N.lib().tg_conic_full_projector(ctypes.byref(call), None)
+    assert rc == N.TG_EINVAL
+    call.map.n[0] = call.map.n[1] = call.map.n[2] = 4
+    call.kind = 7
+    assert N.lib().tg_conic_full_projector(ctypes.byref(call), None) == N.TG_EINVAL
+    assert b"kind" in N.lib().tg_last_error()
+    call.kind = N.TG_PJ
+    call.tile_j = 3
+    assert N.lib().tg_conic_full_projector(ctypes.byref(call), None) == N.TG_EINVAL
+    call.tile_j = 16
+    assert N.lib().tg_conic_image_projector(ctypes.byref(call), 5, None) == N.TG_EINVAL

@@ -1,0 +1,318 @@
+"""Pins the CPU oracle (oracle/siddon_oracle.c) before it is trusted as the checker.
+
+Each check is one the reference's own tests make (tests/test_siddon.py), a
+closed form, or an exact-rational brute-force traversal -- chosen so that a
+dropped term, a wrong sign or index, or a transposed operand fails one of them.
+"""
+import numpy as np
+import pytest
+
+import tomograpy_amd as T
+from oracle import brute
+from tests import cases as C
+
+
+def _project(O, data, cube, sun=False, mask=None, nthread=0):
+    return O.siddon(O.PJ, C.records(data), C.maprec(cube.header), cube, data, sun=sun,
+                    mask=mask, nthread=nthread)
+
+
+def _backproject(O, data, cube, sun=False, mask=None, nthread=0):
+    return O.siddon(O.BPJ, C.records(data), C.maprec(cube.header), cube, data, sun=sun,
+                    mask=mask, nthread=nthread)
+
+
+# ---- tests/test_siddon.py:59-67: unit vectors == the numpy formula --------------------
+@pytest.mark.parametrize("im_idx", [0, 1])
+def test_full_unit_vector_matches_numpy_formula(oracle, im_idx):
+    im_h = C.image_headers(np.float64)[im_idx]
+    data = C.stack(im_h, np.float64, n_images=5)
+    u = oracle.full_unit_vector(C.records(data), data.shape[0], data.shape[1], np.float64)
+    upy = np.empty_like(u)
+    for t, h in enumerate(data.header):
+        R = T.dict_to_array(h, "R")
+        g = (np.arange(data.shape[0]) - h["CRPIX1"] + 1) * h["CDELT1"]
+        l = (np.arange(data.shape[1]) - h["CRPIX2"] + 1) * h["CDELT2"]
+        L, G = np.meshgrid(l, g)
+        u2 = np.stack([np.cos(L) * np.cos(G), np.cos(L) * np.sin(G), np.sin(L)], -1)
+        upy[:, :, t, :] = np.einsum("ij,...j->...i", R, u2)
+    # libm and numpy cos/sin may differ in the last bit; the rotation sums are
+    # evaluated in the same order, so agreement is to a few ulp.
+    np.testing.assert_allclose(u, upy, rtol=0, atol=4e-16)
+
+
+def test_unit_vector_central_ray_points_at_origin(oracle):
+    """Closed form: the reference pixel's ray (gamma = lambda = 0) is -M/|M|."""
+    h = T.centered_image_header(0.01, 2, dtype=np.float64)
+    h["CRPIX1"] = 1.0
+    h["CRPIX2"] = 1.0
+    hs = T.circular_trajectory_header(radius=214.0, longitudes=[0.3, 2.0], latitudes=[0.1, -0.05], **h)
+    data = T.InfoArray(np.zeros((2, 2, 2)), header=hs)
+    u = oracle.full_unit_vector(C.records(data), 2, 2, np.float64)
+    for t, hh in enumerate(hs):
+        M = np.array([hh["M1"], hh["M2"], hh["M3"]])
+        np.testing.assert_allclose(u[0, 0, t], -M / np.linalg.norm(M), atol=1e-15)
+
+
+# ---- tests/test_siddon.py:70-84: a1/an == the numpy formula ---------------------------
+@pytest.mark.parametrize("obj_idx", [0, 1])
+def test_intersection_parameters_match_numpy(oracle, obj_idx):
+    im_h = C.image_headers(np.float64)[1]
+    ch = C.object_headers(np.float64)[obj_idx]
+    data = C.stack(im_h, np.float64, n_images=6)
+    u = oracle.full_unit_vector(C.records(data), 32, 32, np.float64)
+    a1, an = oracle.full_intersection_parameters(C.maprec(ch), C.records(data), u, np.float64)
+    mmin = T.dict_to_array(ch, "MMIN")
+    mmax = T.dict_to_array(ch, "MMAX")
+    with np.errstate(divide="ignore"):
+        for t, h in enumerate(data.header):
+            M = T.dict_to_array(h, "M")
+            for i in range(3):
+                e1 = (mmin[i] - M[i]) / u[..., t, i]
+                en = (mmax[i] - M[i]) / u[..., t, i]
+                e1[np.isinf(e1)] = -T.INF
+                en[np.isinf(en)] = T.INF
+                np.testing.assert_array_equal(a1[..., t, i], e1)
+                np.testing.assert_array_equal(an[..., t, i], en)
+
+
+# ---- exact-rational brute force on tiny cubes -------------------------------------------
+def _well_conditioned(u):
+    a = np.abs(np.asarray(u, dtype=np.float64))
+    return np.all((a == 0) | (a > 1e-6))
+
+
+@pytest.mark.parametrize("n,npix,nv,sun", [(8, 6, 5, False), (5, 7, 9, False), (16, 5, 3, False),
+                                           (8, 9, 4, True), (6, 8, 6, True)])
+def test_traversal_matches_exact_rational(oracle, n, npix, nv, sun):
+    ch = T.centered_cubic_map_header(3.0, n)
+    mrec = C.maprec(ch)
+    # observers close enough that rays cross the sun sphere and many voxel planes
+    ih = T.centered_image_header(T.fov(ch, 4.0), npix)
+    lons = 0.37 + 2 * np.pi * np.arange(nv) / nv
+    hs = T.circular_trajectory_header(radius=4.0, longitudes=lons,
+                                      latitudes=0.2 * np.sin(np.arange(nv)), **ih)
+    recs = T.headers.view_records(hs)
+    checked = 0
+    for t in range(nv):
+        for i in range(npix):
+            for j in range(npix):
+                iv, ln, u = oracle.ray_trace(mrec, (n, n, n), recs[t], i, j, sun=sun)
+                if not _well_conditioned(u):
+                    continue
+                ex = brute.exact_traversal(recs[t][15:18], u, mrec[6:9], mrec[9:12], mrec[0:3],
+                                           (n, n, n), sun=sun)
+                got = [(tuple(int(x) for x in a), l) for a, l in zip(iv, ln) if abs(l) > 1e-9]
+                exs = [((a, b, c), float(l)) for a, b, c, l in ex if abs(l) > 1e-9]
+                assert [g[0] for g in got] == [e[0] for e in exs], (t, i, j)
+                np.testing.assert_allclose([g[1] for g in got], [e[1] for e in exs], atol=1e-12)
+                checked += 1
+    assert checked > nv * npix * npix // 2
+
+
+def test_axis_aligned_and_diagonal_rays_exact(oracle):
+    """Rays along an axis (u components exactly 0) and through voxel corners."""
+    n = 4
+    ch = T.centered_cubic_map_header(4.0, n)  # CDELT = 1, borders -2..2
+    mrec = C.maprec(ch)
+    h = T.centered_image_header(0.5, 3)
+    h["CRPIX1"] = 2.0
+    h["CRPIX2"] = 2.0
+    hs = T.circular_trajectory_header(radius=10.0, longitudes=[0.0], **h)
+    rec = T.headers.view_records(hs)[0]
+    # central pixel (i=j=1): gamma = lambda = 0, u = (-1, 0, 0) exactly, along y = z = 0,
+    # the corner line of four voxel columns.
+    iv, ln, u = oracle.ray_trace(mrec, (n, n, n), rec, 1, 1)
+    assert list(u) == [-1.0, 0.0, 0.0]
+    assert [tuple(x) for x in iv] == [(3, 2, 2), (2, 2, 2), (1, 2, 2), (0, 2, 2)]
+    assert list(ln) == [1.0, 1.0, 1.0, 1.0]
+
+
+def test_sun_stops_before_first_voxel_inside(oracle):
+    """Closed form: a ray through the centre stops at the first voxel whose entry
+    point is within the unit sphere; that voxel is not integrated."""
+    ch = T.centered_cubic_map_header(4.0, 8)  # CDELT = 0.5, borders -2..2
+    h = T.centered_image_header(0.5, 3)
+    h["CRPIX1"] = 2.0
+    h["CRPIX2"] = 2.0
+    hs = T.circular_trajectory_header(radius=10.0, longitudes=[0.0], **h)
+    rec = T.headers.view_records(hs)[0]
+    iv, ln, _ = oracle.ray_trace(C.maprec(ch), (8, 8, 8), rec, 1, 1, sun=True)
+    # entry points x = 2.0, 1.5, 1.0 -> |x|^2 = 4, 2.25, 1: the third fails "> 1"
+    assert [int(x[0]) for x in iv] == [7, 6]
+    assert list(ln) == [0.5, 0.5]
+
+
+# ---- tests/test_siddon.py:88-99: parallel central ray sums to PSHAPE --------------------
+@pytest.mark.parametrize("lon,axis", [(0.0, 0), (np.pi / 2, 1)])
+@pytest.mark.parametrize("obj_idx", [0, 1])
+def test_parallel_ray_sum_is_pshape(oracle, lon, axis, obj_idx):
+    im_h = dict(C.image_headers(np.float64)[0])
+    im_h["CRPIX1"] = 1.0
+    im_h["CRPIX2"] = 1.0
+    data = C.stack(im_h, np.float64, n_images=1, max_lon=lon, min_lon=lon, fill=1.0)
+    ch = C.object_headers(np.float64)[obj_idx]
+    cube = T.InfoArray(np.zeros((ch["NAXIS1"],) * 3), header=dict(ch))
+    _backproject(oracle, data, cube)
+    if lon == 0.0:
+        assert np.sum(cube) == ch["PSHAPE%d" % (axis + 1)]
+    else:
+        # cos(pi/2) != 0 in floating point: the ray is parallel to y up to 6e-17
+        np.testing.assert_allclose(np.sum(cube), ch["PSHAPE%d" % (axis + 1)], rtol=1e-12)
+
+
+# ---- tests/test_siddon.py:104-115: discretisation invariance ----------------------------
+def test_sum_bpj_discretisation_invariance(oracle):
+    data = C.stack(C.image_headers(np.float64)[0], np.float64, n_images=1, fill=1.0)
+    h1, h16 = C.object_headers(np.float64)
+    c16 = T.InfoArray(np.zeros((16, 16, 16)), header=dict(h16))
+    c1 = T.InfoArray(np.zeros((1, 1, 1)), header=dict(h1))
+    _backproject(oracle, data, c16)
+    _backproject(oracle, data, c1)
+    assert np.sum(c16) == c1[0, 0, 0]
+
+
+# ---- tests/test_siddon.py:119-136: linearity ----------------------------------------------
+@pytest.mark.parametrize("dtype", C.DTYPES)
+@pytest.mark.parametrize("obj_idx", [0, 1])
+def test_scale_factor(oracle, dtype, obj_idx):
+    im_h = C.image_headers(dtype)[1]
+    ch = C.object_headers(dtype)[obj_idx]
+    d1 = C.stack(im_h, dtype, n_images=4)
+    d10 = C.stack(im_h, dtype, n_images=4)
+    o1 = T.InfoArray(np.ones((ch["NAXIS1"],) * 3, dtype=dtype), header=dict(ch))
+    o10 = T.InfoArray(10 * np.ones((ch["NAXIS1"],) * 3, dtype=dtype), header=dict(ch))
+    _project(oracle, d1, o1)
+    _project(oracle, d10, o10)
+    np.testing.assert_allclose(10 * np.asarray(d1), np.asarray(d10), rtol=1e-5 if dtype == np.float32 else 1e-12)
+
+
+# ---- tests/test_siddon.py:141-155: data != 0 <=> the ray meets the cube -------------------
+@pytest.mark.parametrize("dtype", C.DTYPES)
+@pytest.mark.parametrize("obj_idx", [0, 1])
+@pytest.mark.parametrize("im_idx", [0, 1])
+def test_special_cases_support(oracle, dtype, obj_idx, im_idx):
+    ch = C.object_headers(dtype)[obj_idx]
+    data = C.stack(C.image_headers(dtype)[im_idx], dtype, n_images=17)
+    cube = T.InfoArray(np.ones((ch["NAXIS1"],) * 3, dtype=dtype), header=dict(ch))
+    _project(oracle, data, cube)
+    recs = C.records(data)
+    u = oracle.full_unit_vector(recs, data.shape[0], data.shape[1], dtype)
+    a1, an = oracle.full_intersection_parameters(C.maprec(ch), recs, u, dtype)
+    lo = np.minimum(a1, an)
+    hi = np.maximum(a1, an)
+    flag = lo.max(axis=-1) < hi.min(axis=-1)
+    assert np.array_equal(np.asarray(data) != 0, flag)
+
+
+# ---- tests/test_siddon.py:41-56: full projection == per-image projection ------------------
+@pytest.mark.parametrize("dtype", C.DTYPES)
+def test_full_versus_image(oracle, dtype):
+    ch, cube, data = C.small_case(dtype=dtype)
+    d2 = data.copy()
+    _project(oracle, data, cube)
+    recs = C.records(d2)
+    for t in range(d2.shape[-1]):
+        oracle.siddon(oracle.PJ, recs, C.maprec(ch), cube, d2, t0=t, t1=t + 1)
+    assert np.array_equal(np.asarray(data), np.asarray(d2))
+
+
+# ---- adjoint identity <P x, y> = <x, P^T y> ----------------------------------------------
+@pytest.mark.parametrize("sun", [False, True])
+def test_adjoint_identity_fp64(oracle, sun):
+    ch, x, data = C.small_case(n=12, npix=20, nv=5, radius=8.0, seed=3)
+    rng = np.random.default_rng(4)
+    y = rng.normal(size=data.shape)
+    px = T.InfoArray(np.zeros(data.shape), header=data.header)
+    _project(oracle, px, x, sun=sun)
+    yy = T.InfoArray(y, header=data.header)
+    pty = T.InfoArray(np.zeros(x.shape), header=x.header)
+    _backproject(oracle, yy, pty, sun=sun)
+    lhs = float(np.sum(np.asarray(px) * y))
+    rhs = float(np.sum(np.asarray(x) * np.asarray(pty)))
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), abs(rhs))
+
+
+def test_sum_of_lengths_is_chord(oracle):
+    """Without obstacle the lengths of a ray telescope to amax - amin."""
+    ch, cube, data = C.small_case(n=10, npix=9, nv=4, radius=6.0)
+    recs = C.records(data)
+    u = oracle.full_unit_vector(recs, 9, 9, np.float64)
+    a1, an = oracle.full_intersection_parameters(C.maprec(ch), recs, u, np.float64)
+    for t in range(4):
+        for i in range(9):
+            for j in range(9):
+                _, ln, _ = oracle.ray_trace(C.maprec(ch), (10, 10, 10), recs[t], i, j)
+                lo = np.minimum(a1[i, j, t], an[i, j, t]).max()
+                hi = np.maximum(a1[i, j, t], an[i, j, t]).min()
+                chord = max(hi - lo, 0.0)
+                assert abs(ln.sum() - chord) < 1e-12
+
+
+def test_mask_skips_rays(oracle):
+    ch, cube, data = C.small_case(n=8, npix=10, nv=3)
+    mask = np.zeros(data.shape, dtype=bool)
+    mask[::2, 1::3, :] = True
+    d_full = data.copy()
+    _project(oracle, d_full, cube)
+    d_m = data.copy()
+    _project(oracle, d_m, cube, mask=mask)
+    assert np.all(np.asarray(d_m)[mask] == 0)
+    assert np.array_equal(np.asarray(d_m)[~mask], np.asarray(d_full)[~mask])
+
+
+def test_time_variant_one_bin_equals_plain(oracle):
+    ch, cube, data = C.small_case(n=8, npix=10, nv=3)
+    d3 = data.copy()
+    _project(oracle, d3, cube)
+    cube4 = np.repeat(np.asarray(cube)[..., None], 3, axis=-1)
+    d4 = data.copy()
+    oracle.siddon(oracle.PJT, C.records(d4), C.maprec(ch), cube4, d4, tindex=np.zeros(3, np.int32))
+    assert np.array_equal(np.asarray(d3), np.asarray(d4))
+
+
+def test_time_variant_bins_are_separable(oracle):
+    """pjt with image t reading bin t (C:931-936) == per-image plain projections."""
+    ch, cube, data = C.small_case(n=8, npix=10, nv=3)
+    rng = np.random.default_rng(9)
+    cube4 = rng.uniform(size=(8, 8, 8, 3))
+    d4 = data.copy()
+    oracle.siddon(oracle.PJT, C.records(d4), C.maprec(ch), cube4, d4)
+    recs = C.records(data)
+    for t in range(3):
+        dt = data.copy()
+        oracle.siddon(oracle.PJ, recs, C.maprec(ch), np.ascontiguousarray(cube4[..., t]), dt,
+                      t0=t, t1=t + 1)
+        assert np.array_equal(np.asarray(dt)[..., t], np.asarray(d4)[..., t])
+    # and the backprojection routes each image to its bin
+    c4 = np.zeros((8, 8, 8, 3))
+    y = T.InfoArray(rng.normal(size=data.shape), header=data.header)
+    oracle.siddon(oracle.BPJT, recs, C.maprec(ch), c4, y, nthread=1)
+    for t in range(3):
+        c = np.zeros((8, 8, 8))
+        oracle.siddon(oracle.BPJ, recs, C.maprec(ch), c, y, t0=t, t1=t + 1)
+        np.testing.assert_allclose(c, c4[..., t], rtol=0, atol=0)
+
+
+def test_bpj_single_thread_is_canonical_order(oracle):
+    """nthread=1 is reproducible bit for bit (the order the deterministic GPU path follows)."""
+    ch, x, data = C.small_case(n=10, npix=16, nv=4, dtype=np.float32)
+    y = T.InfoArray(np.random.default_rng(2).normal(size=data.shape).astype(np.float32), header=data.header)
+    a = np.zeros((10, 10, 10), np.float32)
+    b = np.zeros((10, 10, 10), np.float32)
+    oracle.siddon(oracle.BPJ, C.records(y), C.maprec(ch), a, y, nthread=1)
+    oracle.siddon(oracle.BPJ, C.records(y), C.maprec(ch), b, y, nthread=1)
+    assert np.array_equal(a, b)
+
+
+def test_step_counts_match_traces(oracle):
+    ch, cube, data = C.small_case(n=9, npix=7, nv=3)
+    recs = C.records(data)
+    counts = oracle.ray_steps(C.maprec(ch), (9, 9, 9), recs, 7, 7)
+    for t in range(3):
+        for i in range(7):
+            for j in range(7):
+                iv, _, _ = oracle.ray_trace(C.maprec(ch), (9, 9, 9), recs[t], i, j)
+                assert counts[t, i, j] == len(iv)
+    total = oracle.siddon(oracle.PJ, recs, C.maprec(ch), cube, data.copy())
+    assert total == counts.sum()

@@ -1,0 +1,538 @@
+// gfx950 Siddon conic projector / backprojector: kernels and the C ABI.
+//
+// One ray per lane.  A workgroup of 256 lanes covers a tile of BI x BJ pixels
+// of one view (BJ = 1 << bj_log2 pixels along the image's second axis, which
+// maps to the cube's fastest axis for near-equatorial observers), so the
+// voxel reads of the 64 lanes of a wavefront fall on a few cache lines and the
+// workgroups of one XCD sweep neighbouring rays of the same views.  The view
+// header sits in scalar registers (its address is uniform per workgroup).
+//
+// Forward (pj, pjt): each lane owns its pixel, accumulates in a register in
+// the reference's along-ray order starting from the pixel's current value and
+// stores once, so the result is bit-identical to the reference's (C:915-920,
+// C:931-936), whose per-pixel sums are also sequential.
+// Backward (bpj, bpjt): an atomic scatter (C:923-928, C:939-944), or the
+// fixed-order deterministic path in siddon_det.hip.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "siddon_common.h"
+#include "siddon_host.h"
+
+namespace tg {
+
+// ---------------------------------------------------------------------------
+// Ray-driven kernels
+// ---------------------------------------------------------------------------
+
+// Forward projection (pj / pjt), optionally with the voxel operands of
+// O.P.M (weight, mask) and the pixel weight O.
+template <typename T, bool SUN, bool VOX>
+__global__ __launch_bounds__(256) void k_project(KArgs<T> a) {
+  const uint32_t tile = (a.ray_i >= 0) ? 0u : xcd_remap(blockIdx.x, gridDim.x);
+  int i, j, t;
+  if (!lane_pixel(a, tile, i, j, t)) return;
+  if (a.pixmask && a.pixmask[i * a.ms[0] + j * a.ms[1] + (int64_t)t * a.ms[2]] != 0) return;
+
+  const ViewHdr<T> &h = a.views[t];
+  T u0[3];
+  pixel_unit_vector(a, h, t, i, j, u0);
+  Ray<T> r;
+  if (!ray_setup(a, h.M, u0, r)) return;
+  if (!ray_state_ok(r)) {
+    flag_status(a.status, TG_EGEOMETRY);
+    return;
+  }
+  const int64_t did = i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+  int64_t toff;
+  if (!time_offset(a, t, toff)) return;
+  const T *cube = a.cube + toff;
+  int64_t lin = r.iv[0] * a.cs[0] + r.iv[1] * a.cs[1] + r.iv[2] * a.cs[2];
+  const int64_t st0 = r.up[0] * a.cs[0], st1 = r.up[1] * a.cs[1], st2 = r.up[2] * a.cs[2];
+  int64_t lm = 0, sm0 = 0, sm1 = 0, sm2 = 0;
+  if (VOX) {
+    lm = r.iv[0] * a.vs[0] + r.iv[1] * a.vs[1] + r.iv[2] * a.vs[2];
+    sm0 = r.up[0] * a.vs[0];
+    sm1 = r.up[1] * a.vs[1];
+    sm2 = r.up[2] * a.vs[2];
+  }
+  T acc = a.pixw ? (T)0 : a.data[did];
+  int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
+  while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
+    if (--guard < 0) {
+      flag_status(a.status, TG_EGEOMETRY);
+      break;
+    }
+    bool sx, sy, sz;
+    const T d = step(r, sx, sy, sz);
+    T v = cube[lin];
+    bool keep = true;
+    if (VOX) {
+      if (a.voxw) v = a.voxw[lm] * v;
+      if (a.voxmask) keep = (a.voxmask[lm] == 0);
+      lm += (sx ? sm0 : 0) + (sy ? sm1 : 0) + (sz ? sm2 : 0);
+    }
+    if (keep) acc = acc + d * v;
+    lin += (sx ? st0 : 0) + (sy ? st1 : 0) + (sz ? st2 : 0);
+  }
+  if (a.pixw)
+    a.data[did] = a.data[did] + a.pixw[did] * acc;
+  else
+    a.data[did] = acc;
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T *p, T v) {
+  unsafeAtomicAdd(p, v);
+}
+
+// Backprojection (bpj / bpjt) as an atomic scatter.
+template <typename T, bool SUN, bool VOX>
+__global__ __launch_bounds__(256) void k_backproject(KArgs<T> a) {
+  const uint32_t tile = (a.ray_i >= 0) ? 0u : xcd_remap(blockIdx.x, gridDim.x);
+  int i, j, t;
+  if (!lane_pixel(a, tile, i, j, t)) return;
+  if (a.pixmask && a.pixmask[i * a.ms[0] + j * a.ms[1] + (int64_t)t * a.ms[2]] != 0) return;
+  const int64_t did = i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+  T w = a.data[did];
+  if (a.pixw) w = a.pixw[did] * w;
+  // A zero pixel adds d * 0 to every voxel it crosses: nothing to do.
+  if (w == (T)0) return;
+
+  const ViewHdr<T> &h = a.views[t];
+  T u0[3];
+  pixel_unit_vector(a, h, t, i, j, u0);
+  Ray<T> r;
+  if (!ray_setup(a, h.M, u0, r)) return;
+  if (!ray_state_ok(r)) {
+    flag_status(a.status, TG_EGEOMETRY);
+    return;
+  }
+  int64_t toff;
+  if (!time_offset(a, t, toff)) return;
+  T *cube = a.cube + toff;
+  int64_t lin = r.iv[0] * a.cs[0] + r.iv[1] * a.cs[1] + r.iv[2] * a.cs[2];
+  const int64_t st0 = r.up[0] * a.cs[0], st1 = r.up[1] * a.cs[1], st2 = r.up[2] * a.cs[2];
+  int64_t lm = 0, sm0 = 0, sm1 = 0, sm2 = 0;
+  if (VOX) {
+    lm = r.iv[0] * a.vs[0] + r.iv[1] * a.vs[1] + r.iv[2] * a.vs[2];
+    sm0 = r.up[0] * a.vs[0];
+    sm1 = r.up[1] * a.vs[1];
+    sm2 = r.up[2] * a.vs[2];
+  }
+  int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
+  while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
+    if (--guard < 0) {
+      flag_status(a.status, TG_EGEOMETRY);
+      break;
+    }
+    bool sx, sy, sz;
+    const T d = step(r, sx, sy, sz);
+    bool keep = true;
+    if (VOX) {
+      if (a.voxmask) keep = (a.voxmask[lm] == 0);
+      lm += (sx ? sm0 : 0) + (sy ? sm1 : 0) + (sz ? sm2 : 0);
+    }
+    if (keep) atomic_add(cube + lin, d * w);
+    lin += (sx ? st0 : 0) + (sy ? st1 : 0) + (sz ? st2 : 0);
+  }
+}
+
+// Step count per ray (value independent).
+template <typename T, bool SUN>
+__global__ __launch_bounds__(256) void k_count(KArgs<T> a) {
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  int i, j, t;
+  uint32_t c = 0;
+  bool valid = lane_pixel(a, tile, i, j, t);
+  if (valid && !(a.pixmask && a.pixmask[i * a.ms[0] + j * a.ms[1] + (int64_t)t * a.ms[2]] != 0)) {
+    const ViewHdr<T> &h = a.views[t];
+    T u0[3];
+    pixel_unit_vector(a, h, t, i, j, u0);
+    Ray<T> r;
+    if (ray_setup(a, h.M, u0, r)) {
+      if (!ray_state_ok(r)) {
+        flag_status(a.status, TG_EGEOMETRY);
+      } else {
+        int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
+        while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
+          if (--guard < 0) {
+            flag_status(a.status, TG_EGEOMETRY);
+            break;
+          }
+          bool sx, sy, sz;
+          step(r, sx, sy, sz);
+          c++;
+        }
+      }
+    }
+  }
+  if (valid && a.per_ray)
+    a.per_ray[((size_t)(t - a.t0) * a.n1 + i) * a.n2 + j] = c;
+  if (a.total) {
+    // wavefront reduction, then one atomic per wavefront
+    unsigned long long s = c;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(a.total, s);
+  }
+}
+
+// Unit vectors u[i][j][t][3] (C:618-648); skips pixels whose data is NaN.
+template <typename T>
+__global__ __launch_bounds__(256) void k_unit_vector(KArgs<T> a, T *u) {
+  const uint32_t tile = blockIdx.x;
+  int i, j, t;
+  if (!lane_pixel(a, tile, i, j, t)) return;
+  if (a.data) {
+    const T x = a.data[i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2]];
+    if (x != x) return;
+  }
+  T u0[3];
+  pixel_unit_vector(a, a.views[t], t, i, j, u0);
+  const size_t o = (((size_t)i * a.n2 + j) * a.nv + t) * 3;
+  u[o] = u0[0];
+  u[o + 1] = u0[1];
+  u[o + 2] = u0[2];
+}
+
+// a1/an per pixel from u (C:694-720).
+template <typename T>
+__global__ __launch_bounds__(256) void k_intersections(KArgs<T> a, const T *u, T *a1, T *an) {
+  const uint32_t tile = blockIdx.x;
+  int i, j, t;
+  if (!lane_pixel(a, tile, i, j, t)) return;
+  const size_t o = (((size_t)i * a.n2 + j) * a.nv + t) * 3;
+  T u0[3] = {u[o], u[o + 1], u[o + 2]};
+  T b1[3], bn[3];
+  intersection_parameters(a, a.views[t].M, u0, b1, bn);
+  for (int k = 0; k < 3; k++) {
+    a1[o + k] = b1[k];
+    an[o + k] = bn[k];
+  }
+}
+
+// Elementwise cube *= weight (the voxel half of the Thomson adjoint).
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale_cube(KArgs<T> a) {
+  const int64_t nvox = (int64_t)a.n[0] * a.n[1] * a.n[2];
+  const int64_t total = nvox * a.nt;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = k / nvox, v = k % nvox;
+    const int64_t i0 = v / ((int64_t)a.n[1] * a.n[2]);
+    const int64_t i1 = (v / a.n[2]) % a.n[1];
+    const int64_t i2 = v % a.n[2];
+    T *x = a.cube + i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2] + b * a.cs[3];
+    const T w = a.voxw[i0 * a.vs[0] + i1 * a.vs[1] + i2 * a.vs[2]];
+    *x = w * *x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static thread_local char g_err[1024] = "";
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return TG_OK;
+  return set_error(TG_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// Validates *c and fills the kernel arguments for views [t0, t1).
+template <typename T>
+int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a) {
+  memset(&a, 0, sizeof(a));
+  if (!c) return set_error(TG_EINVAL, "null call");
+  if (c->n1 < 0 || c->n2 < 0 || c->nv < 0)
+    return set_error(TG_EINVAL, "negative image stack shape (%d, %d, %d)", c->n1, c->n2, c->nv);
+  for (int k = 0; k < 3; k++)
+    if (c->map.n[k] <= 0)
+      return set_error(TG_EINVAL, "map shape must be positive, got n[%d] = %d", k, c->map.n[k]);
+  if (c->kind < TG_PJ || c->kind > TG_BPJT) return set_error(TG_EINVAL, "bad kind %d", c->kind);
+  if (t0 < 0 || t1 > c->nv || t0 > t1)
+    return set_error(TG_EINVAL, "view range [%d, %d) outside [0, %d)", t0, t1, c->nv);
+  const bool time = (c->kind == TG_PJT || c->kind == TG_BPJT);
+  const int nt = c->map.nt > 0 ? c->map.nt : 1;
+  if (time && c->map.nt <= 0) return set_error(TG_EINVAL, "time kinds need map.nt >= 1");
+  for (int k = 0; k < 3; k++) {
+    a.mp[k] = (T)c->map.cdelt[k];
+    a.md[k] = (T)c->map.pshape[k];
+    a.mmin[k] = (T)c->map.mmin[k];
+    a.mmax[k] = (T)c->map.mmax[k];
+    a.n[k] = (uint32_t)c->map.n[k];
+  }
+  a.nt = time ? nt : 1;
+  for (int k = 0; k < 4; k++) a.cs[k] = c->map.stride[k];
+  a.n1 = c->n1;
+  a.n2 = c->n2;
+  a.nv = c->nv;
+  a.t0 = t0;
+  a.t1 = t1;
+  a.ray_i = -1;
+  a.ray_j = -1;
+  int tj = c->tile_j > 0 ? c->tile_j : 16;
+  if (tj > 256 || (tj & (tj - 1))) return set_error(TG_EINVAL, "tile_j must be a power of two <= 256");
+  int lg = 0;
+  while ((1 << lg) < tj) lg++;
+  a.bj_log2 = lg;
+  const int bi = 256 >> lg;
+  a.tiles_i = (c->n1 + bi - 1) / bi;
+  a.tiles_j = (c->n2 + tj - 1) / tj;
+  a.kind = c->kind;
+  for (int k = 0; k < 3; k++) {
+    a.ds[k] = c->data_stride[k];
+    a.ms[k] = c->mask_stride[k];
+    a.vs[k] = c->vox_stride[k];
+  }
+  if (!c->views && c->nv > 0) return set_error(TG_EINVAL, "views descriptor pointer is null");
+  a.views = (const ViewHdr<T> *)c->views;
+  a.trig = (const double *)((const char *)c->views + views_hdr_bytes<T>(c->nv));
+  a.cube = (T *)c->cube;
+  a.data = (T *)c->data;
+  a.pixmask = c->pixmask;
+  a.tindex = c->tindex;
+  a.pixw = (const T *)c->pix_weight;
+  a.voxw = (const T *)c->vox_weight;
+  a.voxmask = c->voxmask;
+  a.status = c->status;
+  return TG_OK;
+}
+
+template <typename T>
+uint32_t grid_blocks(const KArgs<T> &a) {
+  return (uint32_t)(a.t1 - a.t0) * (uint32_t)a.tiles_i * (uint32_t)a.tiles_j;
+}
+
+template <typename T, bool SUN, bool VOX>
+void launch_siddon(const KArgs<T> &a, uint32_t blocks, hipStream_t s, bool bwd) {
+  if (bwd)
+    hipLaunchKernelGGL((k_backproject<T, SUN, VOX>), dim3(blocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_project<T, SUN, VOX>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+template <typename T>
+int run_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, hipStream_t s) {
+  KArgs<T> a;
+  int rc = make_args<T>(c, t0, t1, a);
+  if (rc) return rc;
+  if (!c->cube || !c->data) return set_error(TG_EINVAL, "cube and data pointers are required");
+  const bool bwd = (c->kind == TG_BPJ || c->kind == TG_BPJT);
+  if (bwd && (c->flags & TG_DETERMINISTIC)) {
+    if (ray_i >= 0) return set_error(TG_EINVAL, "deterministic mode applies to full/image calls");
+    return run_deterministic<T>(c, a, s);
+  }
+  uint32_t blocks = grid_blocks(a);
+  if (ray_i >= 0) {
+    if (ray_i >= c->n1 || ray_j >= c->n2 || ray_i < 0 || ray_j < 0)
+      return set_error(TG_EINVAL, "pixel (%d, %d) outside the %d x %d image", ray_i, ray_j, c->n1,
+                       c->n2);
+    a.ray_i = ray_i;
+    a.ray_j = ray_j;
+    blocks = 1;
+  }
+  if (blocks == 0) return TG_OK;
+  const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
+  const bool vox = (c->voxmask != nullptr) || (c->vox_weight != nullptr && !bwd);
+  if (sun) {
+    if (vox) launch_siddon<T, true, true>(a, blocks, s, bwd);
+    else launch_siddon<T, true, false>(a, blocks, s, bwd);
+  } else {
+    if (vox) launch_siddon<T, false, true>(a, blocks, s, bwd);
+    else launch_siddon<T, false, false>(a, blocks, s, bwd);
+  }
+  return hip_check(hipGetLastError(), "siddon kernel launch");
+}
+
+int dispatch_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, void *stream) {
+  if (!c) return set_error(TG_EINVAL, "null call");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->dtype == TG_F32) return run_siddon<float>(c, t0, t1, ray_i, ray_j, s);
+  if (c->dtype == TG_F64) return run_siddon<double>(c, t0, t1, ray_i, ray_j, s);
+  return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+template <typename T>
+int run_count(const tg_call *c, uint32_t *per_ray, unsigned long long *total, hipStream_t s) {
+  KArgs<T> a;
+  int rc = make_args<T>(c, 0, c->nv, a);
+  if (rc) return rc;
+  a.per_ray = per_ray;
+  a.total = total;
+  const uint32_t blocks = grid_blocks(a);
+  if (blocks == 0) return TG_OK;
+  if (c->flags & TG_OBSTACLE_SUN)
+    hipLaunchKernelGGL((k_count<T, true>), dim3(blocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_count<T, false>), dim3(blocks), dim3(256), 0, s, a);
+  return hip_check(hipGetLastError(), "count kernel launch");
+}
+
+template <typename T>
+int run_unit(const tg_call *c, int t0, int t1, void *u, hipStream_t s) {
+  KArgs<T> a;
+  int rc = make_args<T>(c, t0, t1, a);
+  if (rc) return rc;
+  if (!u) return set_error(TG_EINVAL, "null u");
+  const uint32_t blocks = grid_blocks(a);
+  if (blocks == 0) return TG_OK;
+  hipLaunchKernelGGL((k_unit_vector<T>), dim3(blocks), dim3(256), 0, s, a, (T *)u);
+  return hip_check(hipGetLastError(), "unit vector kernel launch");
+}
+
+template <typename T>
+int run_inter(const tg_call *c, int t0, int t1, const void *u, void *a1, void *an, hipStream_t s) {
+  KArgs<T> a;
+  int rc = make_args<T>(c, t0, t1, a);
+  if (rc) return rc;
+  if (!u || !a1 || !an) return set_error(TG_EINVAL, "null u/a1/an");
+  const uint32_t blocks = grid_blocks(a);
+  if (blocks == 0) return TG_OK;
+  hipLaunchKernelGGL((k_intersections<T>), dim3(blocks), dim3(256), 0, s, a, (const T *)u, (T *)a1,
+                     (T *)an);
+  return hip_check(hipGetLastError(), "intersection kernel launch");
+}
+
+template <typename T>
+int run_scale(const tg_call *c, hipStream_t s) {
+  KArgs<T> a;
+  tg_call cc = *c;
+  if (cc.kind != TG_PJT && cc.kind != TG_BPJT) cc.map.nt = 1;
+  int rc = make_args<T>(&cc, 0, cc.nv, a);
+  if (rc) return rc;
+  if (!c->cube || !c->vox_weight) return set_error(TG_EINVAL, "scale needs cube and vox_weight");
+  const int64_t total = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.nt;
+  const uint32_t blocks = (uint32_t)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
+  hipLaunchKernelGGL((k_scale_cube<T>), dim3(blocks), dim3(256), 0, s, a);
+  return hip_check(hipGetLastError(), "scale kernel launch");
+}
+
+// ---------------------------------------------------------------------------
+// View packing (host): C:1043-1046 pixel -> angle in T, then libm cos/sin in
+// double as C:1015-1017 calls them.
+// ---------------------------------------------------------------------------
+template <typename T>
+int pack_views(int nv, int n1, int n2, const double *hdr, void *out, size_t bytes) {
+  const size_t need = views_hdr_bytes<T>(nv) + views_trig_bytes(nv, n1, n2);
+  if (bytes < need) return set_error(TG_EINVAL, "views buffer too small: %zu < %zu", bytes, need);
+  if (nv > 0 && !hdr) return set_error(TG_EINVAL, "null header records");
+  memset(out, 0, need);
+  ViewHdr<T> *vh = (ViewHdr<T> *)out;
+  double *trig = (double *)((char *)out + views_hdr_bytes<T>(nv));
+  for (int t = 0; t < nv; t++) {
+    const double *h = hdr + (size_t)t * TG_HDR;
+    ViewHdr<T> &v = vh[t];
+    v.s[0] = (T)h[TG_H_CRPIX1];
+    v.s[1] = (T)h[TG_H_CRPIX2];
+    v.p[0] = (T)h[TG_H_CDELT1];
+    v.p[1] = (T)h[TG_H_CDELT2];
+    v.v[0] = (T)h[TG_H_CRVAL1];
+    v.v[1] = (T)h[TG_H_CRVAL2];
+    for (int k = 0; k < 9; k++) v.R[k] = (T)h[TG_H_R11 + k];
+    for (int k = 0; k < 3; k++) v.M[k] = (T)h[TG_H_M1 + k];
+    double *tr = trig + (size_t)t * ((size_t)n1 + n2) * 2;
+    for (unsigned int i = 0; i < (unsigned)n1; i++) {
+      const T gamma = (i - v.s[0] + 1) * v.p[0] + v.v[0];
+      tr[2 * i] = cos((double)gamma);
+      tr[2 * i + 1] = sin((double)gamma);
+    }
+    for (unsigned int j = 0; j < (unsigned)n2; j++) {
+      const T lambda = (j - v.s[1] + 1) * v.p[1] + v.v[1];
+      tr[2 * ((size_t)n1 + j)] = cos((double)lambda);
+      tr[2 * ((size_t)n1 + j) + 1] = sin((double)lambda);
+    }
+  }
+  return TG_OK;
+}
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" {
+
+const char *tg_last_error(void) { return g_err; }
+
+const char *tg_version(void) { return "tomograpy_amd 0.1.0 gfx950"; }
+
+size_t tg_views_bytes(int32_t dtype, int32_t nv, int32_t n1, int32_t n2) {
+  if (nv < 0 || n1 < 0 || n2 < 0) return 0;
+  if (dtype == TG_F32) return views_hdr_bytes<float>(nv) + views_trig_bytes(nv, n1, n2);
+  if (dtype == TG_F64) return views_hdr_bytes<double>(nv) + views_trig_bytes(nv, n1, n2);
+  return 0;
+}
+
+int tg_views_pack(int32_t dtype, int32_t nv, int32_t n1, int32_t n2, const double *hdr, void *out,
+                  size_t out_bytes) {
+  if (nv < 0 || n1 < 0 || n2 < 0) return set_error(TG_EINVAL, "negative view/image sizes");
+  if (!out) return set_error(TG_EINVAL, "null output buffer");
+  if (dtype == TG_F32) return pack_views<float>(nv, n1, n2, hdr, out, out_bytes);
+  if (dtype == TG_F64) return pack_views<double>(nv, n1, n2, hdr, out, out_bytes);
+  return set_error(TG_EINVAL, "bad dtype %d", dtype);
+}
+
+int tg_conic_full_projector(const tg_call *c, void *stream) {
+  if (!c) return set_error(TG_EINVAL, "null call");
+  return dispatch_siddon(c, 0, c->nv, -1, -1, stream);
+}
+
+int tg_conic_image_projector(const tg_call *c, int32_t t, void *stream) {
+  if (!c) return set_error(TG_EINVAL, "null call");
+  if (t < 0 || t >= c->nv) return set_error(TG_EINVAL, "image index %d outside [0, %d)", t, c->nv);
+  return dispatch_siddon(c, t, t + 1, -1, -1, stream);
+}
+
+int tg_ray_projector(const tg_call *c, int32_t i, int32_t j, int32_t t, void *stream) {
+  if (!c) return set_error(TG_EINVAL, "null call");
+  if (t < 0 || t >= c->nv) return set_error(TG_EINVAL, "image index %d outside [0, %d)", t, c->nv);
+  return dispatch_siddon(c, t, t + 1, i, j, stream);
+}
+
+int tg_unit_vector(const tg_call *c, int32_t t0, int32_t t1, void *u, void *stream) {
+  if (!c) return set_error(TG_EINVAL, "null call");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->dtype == TG_F32) return run_unit<float>(c, t0, t1, u, s);
+  if (c->dtype == TG_F64) return run_unit<double>(c, t0, t1, u, s);
+  return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+int tg_intersection_parameters(const tg_call *c, int32_t t0, int32_t t1, const void *u, void *a1,
+                               void *an, void *stream) {
+  if (!c) return set_error(TG_EINVAL, "null call");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->dtype == TG_F32) return run_inter<float>(c, t0, t1, u, a1, an, s);
+  if (c->dtype == TG_F64) return run_inter<double>(c, t0, t1, u, a1, an, s);
+  return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+int tg_count_steps(const tg_call *c, uint32_t *per_ray, unsigned long long *total, void *stream) {
+  if (!c) return set_error(TG_EINVAL, "null call");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->dtype == TG_F32) return run_count<float>(c, per_ray, total, s);
+  if (c->dtype == TG_F64) return run_count<double>(c, per_ray, total, s);
+  return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+int tg_scale_cube(const tg_call *c, void *stream) {
+  if (!c) return set_error(TG_EINVAL, "null call");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->dtype == TG_F32) return run_scale<float>(c, s);
+  if (c->dtype == TG_F64) return run_scale<double>(c, s);
+  return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+}  // extern "C"

@@ -1,0 +1,28 @@
+// Host-side helpers shared by the translation units of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "siddon_common.h"
+
+namespace tg {
+
+// Records a thread-local error message and returns `code`.
+int set_error(int code, const char *fmt, ...);
+// Maps a HIP error to TG_EHIP with a message; TG_OK on success.
+int hip_check(hipError_t e, const char *what);
+
+// Deterministic backprojection (siddon_det.hip).
+template <typename T>
+int run_deterministic(const tg_call *c, const KArgs<T> &a, hipStream_t s);
+
+// Storage extent (max element offset + 1) a call can touch in the cube.
+inline int64_t cube_extent(const tg_call *c) {
+  const bool time = (c->kind == TG_PJT || c->kind == TG_BPJT);
+  int64_t e = 1;
+  for (int k = 0; k < 3; k++) e += (int64_t)(c->map.n[k] - 1) * c->map.stride[k];
+  if (time) e += (int64_t)((c->map.nt > 0 ? c->map.nt : 1) - 1) * c->map.stride[3];
+  return e;
+}
+
+}  // namespace tg

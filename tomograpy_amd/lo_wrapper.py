@@ -1,0 +1,275 @@
+"""The projector pair as a device-resident linear operator.
+
+Restates tomograpy/lo_wrapper.py:10-73: ``Siddon`` / ``siddon_lo`` (matvec =
+projector into a zeroed image stack, rmatvec = backprojector into a zeroed
+cube, both returning their cached output buffers) and ``Siddon4d`` /
+``siddon4d_lo`` (time-dependent map, ``ng`` images per time group).  The
+reference builds these on the unvendored ``lo`` package; here they are plain
+classes with the same call surface (``P * x``, ``P.T * y``, ``matvec``,
+``rmatvec``, ``shapein``/``shapeout``) whose buffers live on the GPU.
+
+Layouts: the image stack keeps the reference's (n1, n2, n_images) shape but
+is stored view-major (each image contiguous), so a wavefront's pixels are
+adjacent in memory; the cube is C-contiguous (N0, N1, N2), and a 4-D map is
+stored bin-major (T, N0, N1, N2) and presented as (N0, N1, N2, T).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import siddon as S
+from .headers import check_headers, map_record, view_records
+
+
+def _code(dtype):
+    return {torch.float32: N.TG_F32, torch.float64: N.TG_F64,
+            np.dtype("float32"): N.TG_F32, np.dtype("float64"): N.TG_F64}[dtype]
+
+
+def _tdtype(dtype):
+    if isinstance(dtype, torch.dtype):
+        return dtype
+    return {np.dtype("float32"): torch.float32, np.dtype("float64"): torch.float64}[np.dtype(dtype)]
+
+
+def stack_buffer(n1, n2, nv, dtype, dev):
+    """Zeroed (n1, n2, nv) image stack stored view-major."""
+    return torch.zeros((nv, n1, n2), dtype=dtype, device=dev).permute(1, 2, 0)
+
+
+def map4_buffer(n0, n1, n2, nt, dtype, dev):
+    """Zeroed (n0, n1, n2, nt) map stored bin-major."""
+    return torch.zeros((nt, n0, n1, n2), dtype=dtype, device=dev).permute(1, 2, 3, 0)
+
+
+class _Transposed:
+    def __init__(self, op):
+        self.op = op
+
+    def __mul__(self, y):
+        return self.op.rmatvec(y)
+
+    __call__ = __mul__
+
+    @property
+    def T(self):
+        return self.op
+
+
+class Siddon:
+    """Projector pair with fixed geometry (lo_wrapper.py:10-36).
+
+    kwargs forwarded to both directions, as in the reference: ``mask`` (pixels
+    to skip), ``obstacle`` (None / "sun").  Extensions: ``voxmask`` (voxels
+    excluded from the unknowns, the P.Mo^T of models.py:62), ``pix_weight`` /
+    ``vox_weight`` (the diagonals O and M of the Thomson operator O.P.M,
+    models.py:223-235), ``deterministic`` (fixed-order backprojection) and
+    ``dtype``.
+    """
+
+    def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
+                 voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
+                 tile_j=0, device=None):
+        self.data_header = list(data_header)
+        self.map_header = dict(cube_header)
+        check_headers(self.data_header, self.map_header)
+        if dtype is None:
+            from .headers import bitpix
+            dtype = np.dtype(bitpix[str(int(self.data_header[0].get("BITPIX", -64)))])
+        self.tdtype = _tdtype(dtype)
+        self.code = _code(self.tdtype)
+        self.dev = device or S.device()
+        self.n1 = int(self.data_header[0]["NAXIS1"])
+        self.n2 = int(self.data_header[0]["NAXIS2"])
+        self.nv = len(self.data_header)
+        self.cube_shape = tuple(int(self.map_header["NAXIS%d" % (k + 1)]) for k in range(3))
+        self.shapein = self.cube_shape
+        self.shapeout = (self.n1, self.n2, self.nv)
+        self.obstacle = obstacle
+        self.flags = S._OBSTACLES[obstacle]
+        self.deterministic = deterministic
+        self.tile_j = tile_j
+        self.views = S.views_tensor(view_records(self.data_header), self.code, self.n1, self.n2,
+                                    self.dev)
+        self.map_rec = map_record(self.map_header)
+        self.pixmask = S._mask_tensor(mask, self.shapeout, self.dev) if mask is not None else None
+        self.voxmask = None
+        if voxmask is not None:
+            vm = torch.as_tensor(np.asarray(voxmask.cpu() if isinstance(voxmask, torch.Tensor) else voxmask))
+            self.voxmask = (vm != 0).to(torch.uint8).to(self.dev).contiguous()
+        self.pix_weight = None
+        if pix_weight is not None:
+            pw = torch.as_tensor(np.asarray(pix_weight.cpu() if isinstance(pix_weight, torch.Tensor) else pix_weight))
+            buf = stack_buffer(self.n1, self.n2, self.nv, self.tdtype, self.dev)
+            buf.copy_(pw.to(self.dev, self.tdtype))
+            self.pix_weight = buf
+        self.vox_weight = None
+        if vox_weight is not None:
+            vw = torch.as_tensor(np.asarray(vox_weight.cpu() if isinstance(vox_weight, torch.Tensor) else vox_weight))
+            self.vox_weight = vw.to(self.dev, self.tdtype).contiguous()
+        self.xin = torch.zeros(self.cube_shape, dtype=self.tdtype, device=self.dev)
+        self.xout = stack_buffer(self.n1, self.n2, self.nv, self.tdtype, self.dev)
+        self.status = S._status_tensor(self.dev)
+        self._ws = None
+        self._fwd_call = None
+        self._bwd_call = None
+
+    # -- calls -------------------------------------------------------------
+    def _call(self, kind, cube, data):
+        flags = self.flags
+        if kind in (N.TG_BPJ, N.TG_BPJT) and self.deterministic:
+            flags |= N.TG_DETERMINISTIC
+        vw = self.vox_weight if kind in (N.TG_PJ, N.TG_PJT) else None
+        c = S.make_call(kind, self.code, data.shape, data.stride(), cube.shape, cube.stride(),
+                        self.map_rec, self.views, cube.data_ptr(), data.data_ptr(), flags=flags,
+                        tile_j=self.tile_j, pixmask=self.pixmask, pix_weight=self.pix_weight,
+                        vox_weight=vw, voxmask=self.voxmask, status=self.status)
+        if flags & N.TG_DETERMINISTIC:
+            if self._ws is None:
+                self._ws = S.det_workspace(c, self.dev)
+            c.workspace = self._ws.data_ptr()
+            c.workspace_bytes = self._ws.numel()
+        return c
+
+    def _as_cube(self, x):
+        x = torch.as_tensor(x, device=self.dev) if not isinstance(x, torch.Tensor) else x
+        if x.device != self.dev:
+            x = x.to(self.dev)
+        if x.dtype != self.tdtype:
+            raise ValueError("operand dtype %s, operator dtype %s" % (x.dtype, self.tdtype))
+        return x.reshape(self.cube_shape)
+
+    def _as_stack(self, y):
+        y = torch.as_tensor(y, device=self.dev) if not isinstance(y, torch.Tensor) else y
+        if y.device != self.dev:
+            y = y.to(self.dev)
+        if y.dtype != self.tdtype:
+            raise ValueError("operand dtype %s, operator dtype %s" % (y.dtype, self.tdtype))
+        return y.reshape(self.shapeout) if y.dim() != 3 else y
+
+    def project_into(self, x, out):
+        """out += P x (no zeroing)."""
+        x = self._as_cube(x)
+        c = self._call(N.TG_PJ, x, out)
+        N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
+        return out
+
+    def backproject_into(self, y, out):
+        """out += P^T y (no zeroing; the voxel weight of O.P.M is applied to the sum)."""
+        y = self._as_stack(y)
+        if self.vox_weight is not None:
+            tmp = torch.zeros_like(out)
+            c = self._call(N.TG_BPJ, tmp, y)
+            N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
+            sc = S.make_call(N.TG_BPJ, self.code, y.shape, y.stride(), tmp.shape, tmp.stride(),
+                             self.map_rec, self.views, tmp.data_ptr(), y.data_ptr(),
+                             vox_weight=self.vox_weight)
+            N.check(N.lib().tg_scale_cube(ctypes.byref(sc), S.stream_handle()))
+            out.add_(tmp)
+            return out
+        c = self._call(N.TG_BPJ, out, y)
+        N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
+        return out
+
+    def matvec(self, x):
+        """P x into the cached image buffer (zeroed first), lo_wrapper.py:23-28."""
+        self.xout.zero_()
+        return self.project_into(x, self.xout)
+
+    def rmatvec(self, y):
+        """P^T y into the cached cube buffer (zeroed first), lo_wrapper.py:29-34."""
+        self.xin.zero_()
+        return self.backproject_into(y, self.xin)
+
+    def __mul__(self, x):
+        return self.matvec(x)
+
+    __call__ = __mul__
+
+    @property
+    def T(self):
+        return _Transposed(self)
+
+    @property
+    def shape(self):
+        return (int(np.prod(self.shapeout)), int(np.prod(self.shapein)))
+
+    def count_steps(self):
+        """Siddon steps of one application (value independent, cached)."""
+        if getattr(self, "_steps", None) is None:
+            c = self._call(N.TG_PJ, self.xin, self.xout)
+            total = torch.zeros(1, dtype=torch.int64, device=self.dev)
+            per = torch.zeros((self.nv, self.n1, self.n2), dtype=torch.int32, device=self.dev)
+            N.check(N.lib().tg_count_steps(ctypes.byref(c), ctypes.c_void_p(per.data_ptr()),
+                                           ctypes.c_void_p(total.data_ptr()), S.stream_handle()))
+            self._steps = int(total.item())
+            self._rays_traced = int((per > 0).sum().item())
+        return self._steps
+
+
+class Siddon4d(Siddon):
+    """Time-dependent projector pair (lo_wrapper.py:38-67).
+
+    Image ``i + j * ng`` is assigned to time bin ``j`` -- what the reference's
+    loop of projector4d calls over strided slices ``data[..., i::ng]`` produces
+    (lo_wrapper.py:52-55 with C_siddon.c.template:935) -- or bin ``tindex[k]``
+    when an explicit view -> bin map is given.  One launch covers all bins.
+    """
+
+    def __init__(self, data_header, cube_header, ng=1, tindex=None, **kwargs):
+        super().__init__(data_header, cube_header, **kwargs)
+        nt = int(cube_header.get("NAXIS4", 0)) or None
+        if tindex is None:
+            ng = int(ng)
+            tindex = np.arange(self.nv) // ng
+        tindex = np.asarray(tindex, dtype=np.int32)
+        if tindex.shape != (self.nv,):
+            raise ValueError("tindex needs one bin per image")
+        self.nt = int(nt if nt is not None else tindex.max() + 1)
+        if tindex.min() < 0 or tindex.max() >= self.nt:
+            raise ValueError("time bins outside [0, %d)" % self.nt)
+        self.tindex = torch.from_numpy(tindex).to(self.dev)
+        self.shapein = self.cube_shape + (self.nt,)
+        self.xin = map4_buffer(*self.cube_shape, self.nt, self.tdtype, self.dev)
+
+    def _call(self, kind, cube, data):
+        kind = {N.TG_PJ: N.TG_PJT, N.TG_BPJ: N.TG_BPJT}.get(kind, kind)
+        c = super()._call(kind, cube, data)
+        c.map.nt = self.nt
+        c.tindex = self.tindex.data_ptr()
+        return c
+
+    def _as_cube(self, x):
+        x = torch.as_tensor(x, device=self.dev) if not isinstance(x, torch.Tensor) else x
+        if x.dtype != self.tdtype:
+            raise ValueError("operand dtype %s, operator dtype %s" % (x.dtype, self.tdtype))
+        return x.reshape(self.shapein) if x.dim() != 4 else x
+
+    def backproject_into(self, y, out):
+        y = self._as_stack(y)
+        if self.vox_weight is not None:
+            raise NotImplementedError("voxel weights with time bins")
+        c = self._call(N.TG_BPJ, out, y)
+        N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
+        return out
+
+    def count_steps(self):
+        if getattr(self, "_steps", None) is None:
+            c = self._call(N.TG_PJ, self.xin, self.xout)
+            total = torch.zeros(1, dtype=torch.int64, device=self.dev)
+            N.check(N.lib().tg_count_steps(ctypes.byref(c), None,
+                                           ctypes.c_void_p(total.data_ptr()), S.stream_handle()))
+            self._steps = int(total.item())
+        return self._steps
+
+
+def siddon_lo(data_header, cube_header, **kwargs):
+    """lo_wrapper.py:69-70"""
+    return Siddon(data_header, cube_header, **kwargs)
+
+
+def siddon4d_lo(data_header, cube_header, **kwargs):
+    """lo_wrapper.py:72-73"""
+    return Siddon4d(data_header, cube_header, **kwargs)

@@ -155,14 +155,11 @@ def main():
 
     extra = {}
     if args.cg_iters:
-        A = CG.NormalOperator(P, lam=1e-2, identity=True)
-        bvec = torch.zeros(P.shapein, dtype=tdt, device="cuda")
-        P.backproject_into(y, bvec)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        _, info = CG.cg(A, bvec, maxiter=args.cg_iters)
+        _, info = CG.solve(P, y, lam=1e-2, identity=True, maxiter=args.cg_iters)
         e1.record(stream)
         torch.cuda.synchronize()
         extra["cg"] = {"iterations": info["iterations"], "ms_per_iteration":

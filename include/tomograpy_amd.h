@@ -4,8 +4,8 @@
  *
  * Every entry point is a plain C function over POD structs, raw device
  * pointers and sizes.  The caller owns every buffer (the Python layer uses the
- * torch allocator); the library allocates nothing on the device except inside
- * tg_batch_* (documented there).  Calls are asynchronous on the caller's HIP
+ * torch allocator); the library allocates nothing on the device.  Calls are
+ * asynchronous on the caller's HIP
  * stream and return 0 on success or a negative TG_E* code; tg_last_error()
  * returns a thread-local message for the last failing call on this thread.
  *
@@ -38,6 +38,8 @@ extern "C" {
 #define TG_OBSTACLE_SUN 1u   /* stop a ray at the unit sphere (C:956-961) */
 #define TG_DETERMINISTIC 2u  /* bpj/bpjt: sum each voxel in the reference's serial
                                 order (view, row, column), bit-exact to nthread=1 */
+#define TG_ZERO_OUT 4u       /* pj/pjt: write every pixel of the launched views as if the
+                                image stack had been zeroed first (lo_wrapper.py:26) */
 
 /* Error codes */
 #define TG_OK 0
@@ -167,6 +169,62 @@ int tg_cg_dot(int32_t dtype, int64_t n, const void *a, const void *b, double *sc
 int tg_cg_update(int32_t dtype, int64_t n, void *x, void *r, const void *p, const void *q,
                  double *scal, void *stream);
 int tg_cg_direction(int32_t dtype, int64_t n, const void *r, void *p, double *scal, void *stream);
+
+/* ---- segmented CG: the vector work of many concurrent solves in one launch
+ * (no reference counterpart; used by the reconstruction server) ---- */
+
+/* One solve's vectors (contiguous, n elements each) and its regulariser. */
+typedef struct tg_cg_seg {
+  void *x, *r, *p, *q;
+  const uint8_t *voxmask; /* may be NULL */
+  double *scal;           /* this solve's CG scalars, layout of tg_cg_update */
+  int64_t n;
+  int64_t block0;         /* first launch block of this segment: sum of tg_cg_seg_blocks(n) before it */
+  int32_t shape[4];       /* regulariser array shape (C-contiguous, trailing 1s) */
+  uint32_t axes;          /* regulariser difference axes (bit k = axis k) */
+  int32_t identity;       /* add the identity term */
+  double lam;
+} tg_cg_seg;
+
+#define TG_CG_OP_REGULARIZE 0 /* q += lam ((identity ? p : 0) + sum_k D_k^T D_k p) */
+#define TG_CG_OP_DOT_PQ 1     /* scal[1] = p.q */
+#define TG_CG_OP_UPDATE 2     /* alpha = scal[0]/scal[1]; x += alpha p; r -= alpha q; scal[2] = r.r */
+#define TG_CG_OP_DIRECTION 3  /* beta = scal[2]/scal[0]; p = r + beta p; scal[0] = scal[2] */
+#define TG_CG_OP_INIT 4       /* scal[0] = scal[5] = r.r */
+#define TG_CG_OP_ZERO_Q 5     /* q = 0 */
+
+/* Launch blocks a segment of n elements takes. */
+int64_t tg_cg_seg_blocks(int64_t n);
+
+/* Applies `op` to every segment of the device table segs[0..nseg) in one
+ * launch of nblocks blocks (the sum of the segments' tg_cg_seg_blocks);
+ * `partials` is device scratch of nblocks doubles.  Reductions run in a fixed
+ * order per segment. */
+int tg_cg_batch(int32_t dtype, int32_t op, const tg_cg_seg *segs, int32_t nseg, int64_t nblocks,
+                double *partials, void *stream);
+
+/* ---- batched uneven jobs in one persistent launch (no reference counterpart:
+ * the reference runs one conic_full_projector call per job) ---- */
+
+/* Schedule of a batched launch. */
+typedef struct tg_batch_plan {
+  int32_t tile_j;   /* pixels per workgroup row for every job (power of two); 0 = each job's */
+  int32_t blocks;   /* persistent workgroups; 0 = 8 per compute unit */
+  int32_t schedule; /* 0 = jobs in the given order, 1 = longest rays first */
+  int32_t pad;
+} tg_batch_plan;
+
+/* Bytes of the device queue buffer (and of the host staging buffer) for njobs. */
+size_t tg_batch_queue_bytes(int32_t njobs);
+
+/* Runs calls[0..njobs) -- all TG_PJ/TG_PJT or all TG_BPJ/TG_BPJT, one dtype,
+ * per-job geometry, sizes, obstacle and operands -- as one persistent launch.
+ * The job table is written to `host_staging` (tg_batch_queue_bytes bytes,
+ * caller-owned; keep it unchanged until the stream has passed this call) and
+ * copied to `queue` (device, same size, caller-owned) on `stream`.  Each job's
+ * result equals its one-job call's (the forward projector bit for bit). */
+int tg_batch_run(const tg_call *calls, int32_t njobs, const tg_batch_plan *plan, void *queue,
+                 void *host_staging, void *stream);
 
 /* Thread-local message describing the last error on this thread. */
 const char *tg_last_error(void);

@@ -1,0 +1,108 @@
+"""GPU checks of the batched persistent launch, the autotuner and the
+continuous-batching server: every job/request gives the result it gives alone."""
+import numpy as np
+import pytest
+import torch
+
+from tomograpy_amd import batch as B
+from tomograpy_amd import models as Mo
+from tomograpy_amd import serve as SV
+from tomograpy_amd import workloads as W
+from tomograpy_amd.lo_wrapper import Siddon
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def small_jobs(n_jobs=10, seed=0):
+    rng = np.random.default_rng(seed)
+    ops, xs = [], []
+    for k in range(n_jobs):
+        n = int(rng.integers(12, 41))
+        nv = int(rng.integers(3, 9))
+        npx = int(rng.choice([16, 24, 48]))
+        d = float(rng.uniform(150, 250))
+        ch = W.cube_header(n)
+        hs = W.view_headers(npx, rng.uniform(0, 2 * np.pi, nv), np.deg2rad(rng.uniform(-7, 7, nv)), d=d)
+        if k % 2:
+            P = Mo.thomson_lo(hs, ch, dtype=np.float32)
+        else:
+            P = Siddon(hs, ch, dtype=np.float32, obstacle="sun" if k % 4 == 0 else None)
+        ops.append(P)
+        xs.append(torch.from_numpy(W.blob_phantom(ch, 4, rng)).cuda())
+    return ops, xs
+
+
+@pytest.mark.parametrize("tile_j,schedule", [(16, 0), (8, 1), (32, 1), (64, 0)])
+def test_batched_forward_bit_exact_and_backward_close(tile_j, schedule):
+    print("batch", tile_j, schedule, flush=True)
+    ops, xs = small_jobs()
+    singles = [P.matvec(x).clone() for P, x in zip(ops, xs)]
+    outs = [torch.zeros_like(s) for s in singles]
+    B.run(ops, xs, outs, backward=False, plan=B.make_plan(tile_j=tile_j, schedule=schedule))
+    torch.cuda.synchronize()
+    for s, o in zip(singles, outs):
+        assert torch.equal(s, o)
+    # backward: images -> cubes, against the one-job calls
+    ys = [torch.rand_like(s) for s in singles]
+    sb = [P.rmatvec(y).clone() for P, y in zip(ops, ys)]
+    ob = [torch.zeros_like(b) for b in sb]
+    B.run(ops, ys, ob, backward=True, plan=B.make_plan(tile_j=tile_j, schedule=schedule))
+    torch.cuda.synchronize()
+    for s, o in zip(sb, ob):
+        assert rel_l2(o.cpu(), s.cpu()) <= 1e-5
+
+
+def test_zero_out_matches_zeroed_accumulate():
+    ops, xs = small_jobs(4, seed=3)
+    ref = [P.matvec(x).clone() for P, x in zip(ops, xs)]
+    outs = [torch.full_like(r, 7.0) for r in ref]  # garbage that zero_out must overwrite
+    B.run(ops, xs, outs, backward=False, zero_out=True)
+    for r, o in zip(ref, outs):
+        assert torch.equal(r, o)
+
+
+def test_autotuner_picks_a_plan_and_keeps_results():
+    ops, xs = small_jobs(6, seed=1)
+    tuner = B.Autotuner(tiles=(8, 16, 32), schedules=(0, 1), repeats=1)
+    outs = [torch.zeros((P.n1, P.n2, P.nv), dtype=torch.float32, device="cuda") for P in ops]
+    plan = tuner.plan_for(ops, xs, outs)
+    assert plan.tile_j in (8, 16, 32) and plan.schedule in (0, 1)
+    assert len(tuner.trials[B.signature(ops, False)]) == 6
+    assert all(float(o.abs().sum()) == 0.0 for o in outs)  # tuning restores the outputs
+    B.run_tuned(tuner, ops, xs, outs)
+    for P, x, o in zip(ops, xs, outs):
+        assert torch.equal(P.matvec(x), o)
+
+
+def _requests(n, seed, rate=None):
+    rng = np.random.default_rng(seed)
+    mix = W.config5_mix(n, rng, n_range=(16, 40), v_range=(4, 12), pix=(16, 32, 48), max_bins=3)
+    return SV.make_requests(mix, rng, rate=rate, tol=1e-4, maxiter=300)
+
+
+def test_served_results_match_plain():
+    """Each reconstruction is the same alone (plain) and in the mix (served):
+    within 1e-4 relative L2, both converged to the tolerance."""
+    reqs = _requests(10, seed=2)
+    plain = _requests(10, seed=2)
+    server = SV.Server(budget_bytes=1 << 30, max_active=8, check_every=5)
+    s_sum = server.run(reqs)
+    p_sum = SV.serve_plain(plain, check_every=5)
+    assert s_sum["requests"] == p_sum["requests"] == 10
+    for a, b in zip(reqs, plain):
+        assert a.residual <= 1e-4 and b.residual <= 1e-4
+        assert rel_l2(a.x, b.x) <= 1e-4, (a.id, rel_l2(a.x, b.x))
+    assert s_sum["p99_s"] >= s_sum["p50_s"] > 0
+
+
+def test_admission_rejects_oversize_request():
+    reqs = _requests(1, seed=4)
+    server = SV.Server(budget_bytes=1024, max_active=2)
+    with pytest.raises(ValueError, match="budget"):
+        server.run(reqs)

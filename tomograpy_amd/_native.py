@@ -14,6 +14,7 @@ TG_F32, TG_F64 = 0, 1
 TG_PJ, TG_BPJ, TG_PJT, TG_BPJT = 0, 1, 2, 3
 TG_OBSTACLE_SUN = 1
 TG_DETERMINISTIC = 2
+TG_ZERO_OUT = 4
 TG_HDR = 18
 TG_CG_SCAL = 16 + 1024
 
@@ -44,6 +45,16 @@ class TgCall(ctypes.Structure):
                 ("status", _vp), ("workspace", _vp), ("workspace_bytes", _sz)]
 
 
+class TgCgSeg(ctypes.Structure):
+    _fields_ = [("x", _vp), ("r", _vp), ("p", _vp), ("q", _vp), ("voxmask", _vp), ("scal", _vp),
+                ("n", _i64), ("block0", _i64), ("shape", _i32 * 4), ("axes", _u32),
+                ("identity", _i32), ("lam", _dbl)]
+
+
+(TG_CG_OP_REGULARIZE, TG_CG_OP_DOT_PQ, TG_CG_OP_UPDATE, TG_CG_OP_DIRECTION, TG_CG_OP_INIT,
+ TG_CG_OP_ZERO_Q) = range(6)
+
+
 class TgBatchPlan(ctypes.Structure):
     _fields_ = [("tile_j", _i32), ("blocks", _i32), ("schedule", _i32), ("pad", _i32)]
 
@@ -67,6 +78,11 @@ SIGNATURES = [
     ("tg_cg_dot", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _i32, _vp]),
     ("tg_cg_update", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("tg_cg_direction", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp]),
+    ("tg_cg_seg_blocks", _i64, [_i64]),
+    ("tg_cg_batch", ctypes.c_int, [_i32, _i32, _vp, _i32, _i64, _vp, _vp]),
+    ("tg_batch_queue_bytes", _sz, [_i32]),
+    ("tg_batch_run", ctypes.c_int, [ctypes.POINTER(TgCall), _i32, ctypes.POINTER(TgBatchPlan),
+                                    _vp, _vp, _vp]),
 ]
 
 

@@ -1,0 +1,142 @@
+"""Many uneven projector / backprojector jobs in one persistent launch, with
+an autotuner for the ray-tile shape and the schedule.
+
+A job is a ``lo_wrapper.Siddon`` operator plus its operands; jobs may differ
+in cube size, view count, image size, observer distance, obstacle and
+Thomson weights.  ``run`` launches all of them at once (``tg_batch_run``);
+each job's output equals its one-job call's (the forward projection bit for
+bit).  ``Autotuner`` times the candidate plans on the actual batch and keeps
+the fastest per batch signature.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import siddon as S
+
+TILE_CHOICES = (8, 16, 32, 64)
+SCHEDULES = (0, 1)
+
+
+class _Buffers:
+    """Device queue + pinned host staging for a batch of up to njobs jobs."""
+
+    def __init__(self, njobs, dev):
+        nbytes = int(N.lib().tg_batch_queue_bytes(njobs))
+        self.njobs = njobs
+        self.queue = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.stage = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+
+
+_BUFFERS = {}
+
+
+def _buffers(njobs, dev):
+    key = (str(dev),)
+    b = _BUFFERS.get(key)
+    if b is None or b.njobs < njobs:
+        b = _Buffers(max(njobs, 16), dev)
+        _BUFFERS[key] = b
+    return b
+
+
+def make_plan(tile_j=0, blocks=0, schedule=1):
+    p = N.TgBatchPlan()
+    p.tile_j = int(tile_j)
+    p.blocks = int(blocks)
+    p.schedule = int(schedule)
+    return p
+
+
+def run(ops, ins, outs, backward=False, plan=None, stream=None, zero_out=False):
+    """outs[k] += P_k ins[k] (or P_k^T ins[k] when backward) for all k, one launch.
+
+    ops: Siddon operators (same dtype); ins/outs: device tensors shaped like the
+    operator's cube / image stack (outputs are accumulated into, as in the
+    one-job calls; with zero_out=True a forward launch overwrites them as if
+    they had been zeroed).  Voxel weights of a backward Thomson job are applied
+    to its output after the launch.
+    """
+    n = len(ops)
+    if n == 0:
+        return outs
+    calls = (N.TgCall * n)()
+    post = []
+    for k, (P, x, y) in enumerate(zip(ops, ins, outs)):
+        if backward:
+            target = y
+            if P.vox_weight is not None:
+                target = torch.zeros_like(y)
+                post.append((P, target, y))
+            data = P._as_stack(x)
+            c = P._call(N.TG_BPJ, target, data)
+        else:
+            c = P._call(N.TG_PJ, P._as_cube(x), y)
+            if zero_out:
+                c.flags |= N.TG_ZERO_OUT
+        ctypes.memmove(ctypes.byref(calls[k]), ctypes.byref(c), ctypes.sizeof(N.TgCall))
+    bufs = _buffers(n, ops[0].dev)
+    plan = plan if plan is not None else make_plan()
+    s = S.stream_handle() if stream is None else stream
+    N.check(N.lib().tg_batch_run(calls, n, ctypes.byref(plan), bufs.queue.data_ptr(),
+                                 bufs.stage.data_ptr(), s))
+    # the staging buffer is reused by the next launch: wait until the copy left it
+    torch.cuda.current_stream().synchronize()
+    for P, tmp, y in post:
+        sc = S.make_call(N.TG_BPJ, P.code, P.xout.shape, P.xout.stride(), tmp.shape, tmp.stride(),
+                         P.map_rec, P.views, tmp.data_ptr(), P.xout.data_ptr(),
+                         vox_weight=P.vox_weight)
+        N.check(N.lib().tg_scale_cube(ctypes.byref(sc), S.stream_handle()))
+        y.add_(tmp)
+    return outs
+
+
+def signature(ops, backward):
+    return (bool(backward),) + tuple((P.cube_shape, P.shapeout, P.flags, P.pix_weight is not None,
+                                      P.vox_weight is not None, P.voxmask is not None) for P in ops)
+
+
+class Autotuner:
+    """Picks the (tile_j, schedule) plan with the lowest measured launch time."""
+
+    def __init__(self, tiles=TILE_CHOICES, schedules=SCHEDULES, repeats=2):
+        self.tiles = tiles
+        self.schedules = schedules
+        self.repeats = repeats
+        self.best = {}
+        self.trials = {}
+
+    def plan_for(self, ops, ins, outs, backward=False, zero_out=False):
+        key = signature(ops, backward)
+        if key in self.best:
+            return self.best[key]
+        saved = [o.clone() for o in outs]
+        results = []
+        for tj in self.tiles:
+            for sch in self.schedules:
+                plan = make_plan(tile_j=tj, schedule=sch)
+                run(ops, ins, outs, backward, plan, zero_out=zero_out)  # warm-up
+                torch.cuda.synchronize()
+                t = []
+                for _ in range(self.repeats):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    run(ops, ins, outs, backward, plan, zero_out=zero_out)
+                    e1.record()
+                    torch.cuda.synchronize()
+                    t.append(e0.elapsed_time(e1))
+                results.append((min(t), tj, sch))
+        for o, s0 in zip(outs, saved):
+            o.copy_(s0)
+        results.sort()
+        _, tj, sch = results[0]
+        self.trials[key] = results
+        self.best[key] = make_plan(tile_j=tj, schedule=sch)
+        return self.best[key]
+
+
+def run_tuned(tuner, ops, ins, outs, backward=False):
+    return run(ops, ins, outs, backward, tuner.plan_for(ops, ins, outs, backward))

@@ -3,12 +3,16 @@
     (P^T P + lam (I + sum_k D_k^T D_k)) x = P^T y
 
 the system lo.acg solves for the reference's models (srt.py:225,
-exemples/test_siddon_simu_lo.py:31, models.py:85-94 for the lo.diff priors).
+exemples/test_siddon_simu_lo.py:31; the priors lo.diff of models.py:85-94).
 ``lo`` is not vendored, so its stopping rule and hyper-parameter scaling are
-unpinned; this is textbook CG (Hestenes-Stiefel) with the residual test
-||r|| <= tol ||r0||.  Every vector stays on the GPU; the CG scalars live in a
-device array, so a fixed-iteration solve makes no host round trip and a
-residual-stopping solve synchronises once every ``check_every`` iterations.
+unpinned; this is textbook CG (Hestenes-Stiefel) stopped on ||r|| <= tol ||r0||.
+
+Every vector and scalar stays on the GPU.  One solve and many concurrent
+solves (the reconstruction server) run the same kernels: the segmented CG
+kernels (``tg_cg_batch``) take a table of solves, and the projector pair runs
+as one-job launches for one solve or one batched launch for many.  A solve
+therefore computes the same iterates alone or in a mix, up to the summation
+order of the atomic backprojection.
 """
 import ctypes
 
@@ -16,11 +20,170 @@ import numpy as np
 import torch
 
 from . import _native as N
+from . import batch as B
 from . import siddon as S
+from .lo_wrapper import Siddon4d
+
+
+class CGSolve:
+    """Vectors, scalars and regulariser of one solve on the device.
+
+    Vectors are stored contiguously in the operator's storage layout: the cube
+    (N0, N1, N2), or (T, N0, N1, N2) for a time-binned operator (presented to
+    the projector as (N0, N1, N2, T)).
+    """
+
+    def __init__(self, P, y, lam=1e-2, identity=True, axes=None, tol=1e-4, maxiter=50,
+                 scal=None):
+        self.P = P
+        self.lam = float(lam)
+        self.identity = bool(identity)
+        self.tol = tol
+        self.maxiter = int(maxiter)
+        self.iterations = 0
+        self.done = False
+        dt, dev = P.tdtype, P.dev
+        self.time = isinstance(P, Siddon4d)
+        if self.time:
+            n0, n1, n2 = P.cube_shape
+            self.store_shape = (P.nt, n0, n1, n2)
+            # user axes refer to (N0, N1, N2, T); storage puts T first
+            ax = tuple(range(4)) if axes is None else tuple(axes)
+            self.store_axes = tuple((a + 1) if a < 3 else 0 for a in ax)
+        else:
+            self.store_shape = tuple(P.cube_shape)
+            self.store_axes = tuple(range(3)) if axes is None else tuple(axes)
+        self.n = int(np.prod(self.store_shape))
+        mk = lambda: torch.zeros(self.store_shape, dtype=dt, device=dev)
+        self.xs, self.rs, self.ps, self.qs = mk(), mk(), mk(), mk()
+        self.scal = scal if scal is not None else torch.zeros(16, dtype=torch.float64, device=dev)
+        self.scal.zero_()
+        self.mask = None
+        vm = getattr(P, "voxmask", None)
+        if vm is not None:
+            self.mask = vm.expand(self.store_shape).contiguous() if self.time else vm
+        self.y = P._as_stack(y)
+        self.out = P.xout  # forward output buffer of this solve
+        # r0 = P^T y (x0 = 0), p0 = r0
+        P.backproject_into(self.y, self.view(self.rs))
+        self.ps.copy_(self.rs)
+
+    def view(self, store):
+        """The projector's view of a storage tensor."""
+        return store.permute(1, 2, 3, 0) if self.time else store
+
+    @property
+    def x(self):
+        return self.view(self.xs)
+
+    def seg(self, block0):
+        g = N.TgCgSeg()
+        g.x, g.r, g.p, g.q = (t.data_ptr() for t in (self.xs, self.rs, self.ps, self.qs))
+        if self.mask is not None:
+            g.voxmask = self.mask.data_ptr()
+        g.scal = self.scal.data_ptr()
+        g.n = self.n
+        g.block0 = int(block0)
+        shp = list(self.store_shape) + [1] * (4 - len(self.store_shape))
+        for k in range(4):
+            g.shape[k] = int(shp[k])
+        bits = 0
+        for a in self.store_axes:
+            bits |= 1 << a
+        g.axes = bits
+        g.identity = int(self.identity)
+        g.lam = self.lam
+        return g
+
+    def residual(self, scal_host=None):
+        v = self.scal.cpu().numpy() if scal_host is None else scal_host
+        return float(np.sqrt(v[0] / v[5])) if v[5] > 0 else 0.0
+
+
+class SegTable:
+    """Device table of tg_cg_seg for a list of solves, plus reduction scratch."""
+
+    def __init__(self, solves, dev):
+        lib = N.lib()
+        self.nseg = len(solves)
+        segs = (N.TgCgSeg * self.nseg)()
+        block = 0
+        for k, s in enumerate(solves):
+            g = s.seg(block)
+            ctypes.memmove(ctypes.byref(segs[k]), ctypes.byref(g), ctypes.sizeof(N.TgCgSeg))
+            block += int(lib.tg_cg_seg_blocks(s.n))
+        self.nblocks = block
+        raw = ctypes.string_at(ctypes.addressof(segs), ctypes.sizeof(segs))
+        host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
+        self.dev_table = host.to(dev)
+        self.partials = torch.empty(max(block, 1), dtype=torch.float64, device=dev)
+        self.code = solves[0].P.code
+
+    def op(self, op):
+        N.check(N.lib().tg_cg_batch(self.code, op, self.dev_table.data_ptr(), self.nseg,
+                                    self.nblocks, self.partials.data_ptr(), S.stream_handle()))
+
+
+def _forward_one(s):
+    P = s.P
+    c = P._call(N.TG_PJ, s.view(s.ps), s.out)
+    c.flags |= N.TG_ZERO_OUT
+    N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
+
+
+def _backward_one(s):
+    s.P.backproject_into(s.out, s.view(s.qs))
+
+
+def iterate(solves, table, tuner=None):
+    """One CG iteration of every solve in `solves` (table built over the same list)."""
+    table.op(N.TG_CG_OP_ZERO_Q)
+    if len(solves) == 1:
+        _forward_one(solves[0])
+        _backward_one(solves[0])
+    else:
+        ops = [s.P for s in solves]
+        ins = [s.view(s.ps) for s in solves]
+        outs = [s.out for s in solves]
+        plan_f = tuner.plan_for(ops, ins, outs, False, zero_out=True) if tuner else None
+        B.run(ops, ins, outs, backward=False, plan=plan_f, zero_out=True)
+        qs = [s.view(s.qs) for s in solves]
+        plan_b = tuner.plan_for(ops, outs, qs, True) if tuner else None
+        B.run(ops, outs, qs, backward=True, plan=plan_b)
+    table.op(N.TG_CG_OP_REGULARIZE)
+    table.op(N.TG_CG_OP_DOT_PQ)
+    table.op(N.TG_CG_OP_UPDATE)
+    table.op(N.TG_CG_OP_DIRECTION)
+    for s in solves:
+        s.iterations += 1
+
+
+def solve(P, y, lam=1e-2, identity=True, axes=None, maxiter=50, tol=None, check_every=10,
+          callback=None):
+    """Solves the regularised normal equations for data y (device, P.shapeout).
+
+    Returns (x, info): x in the projector's layout, info = {"iterations",
+    "residual" (||r|| / ||r0||)}.
+    """
+    s = CGSolve(P, y, lam=lam, identity=identity, axes=axes, tol=tol, maxiter=maxiter)
+    table = SegTable([s], P.dev)
+    table.op(N.TG_CG_OP_INIT)
+    for it in range(1, maxiter + 1):
+        iterate([s], table)
+        if callback is not None:
+            callback(it, s.x)
+        if tol is not None and (it % check_every == 0 or it == maxiter):
+            if s.residual() <= tol:
+                break
+    return s.x, {"iterations": s.iterations, "residual": s.residual()}
+
+
+reconstruct = solve
 
 
 class NormalOperator:
-    """x -> P^T P x + lam ((x if identity) + sum_k D_k^T D_k x) on the device."""
+    """x -> P^T P x + lam ((x if identity) + sum_k D_k^T D_k x): the operator
+    ``solve`` applies, for use outside a solve (benchmarks, checks)."""
 
     def __init__(self, P, lam=1e-2, identity=True, axes=None):
         self.P = P
@@ -28,86 +191,23 @@ class NormalOperator:
         self.identity = bool(identity)
         nd = len(P.shapein)
         self.axes = tuple(range(nd)) if axes is None else tuple(axes)
-        self.mask_bits = 0
+        bits = 0
         for a in self.axes:
-            self.mask_bits |= 1 << a
+            bits |= 1 << a
+        self.mask_bits = bits
         self.shape4 = (ctypes.c_int32 * 4)(*(list(P.shapein) + [1] * (4 - nd)))
-        self.code = P.code
-        self._y = None
 
     def apply_into(self, x, out):
-        """out = A x (out is overwritten)."""
+        """out = A x for contiguous x, out shaped like P.shapein."""
         P = self.P
-        y = P.xout
-        y.zero_()
-        P.project_into(x, y)
+        c = P._call(N.TG_PJ, x, P.xout)
+        c.flags |= N.TG_ZERO_OUT
+        N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
         out.zero_()
-        P.backproject_into(y, out)
+        P.backproject_into(P.xout, out)
         if self.lam != 0.0:
-            xc = x if x.is_contiguous() else x.contiguous()
-            oc = out
-            if not out.is_contiguous():
-                raise ValueError("regularizer output must be contiguous")
             vm = P.voxmask.data_ptr() if getattr(P, "voxmask", None) is not None else None
-            N.check(N.lib().tg_regularize(self.code, self.shape4, self.mask_bits, self.lam,
-                                          int(self.identity), xc.data_ptr(), oc.data_ptr(),
-                                          vm, S.stream_handle()))
+            N.check(N.lib().tg_regularize(P.code, self.shape4, self.mask_bits, self.lam,
+                                          int(self.identity), x.data_ptr(), out.data_ptr(), vm,
+                                          S.stream_handle()))
         return out
-
-
-def _flat(t):
-    return t.reshape(-1)
-
-
-def cg(A, b, x0=None, maxiter=50, tol=None, check_every=10, callback=None):
-    """Solves A x = b with A a NormalOperator.  b: device tensor shaped like the cube.
-
-    Returns (x, info) with info = {"iterations", "residual" (||r||/||r0||)}.
-    """
-    P = A.P
-    code = A.code
-    dev = b.device
-    n = b.numel()
-    shape = tuple(P.shapein)
-    x = torch.zeros(shape, dtype=b.dtype, device=dev) if x0 is None else x0.clone().reshape(shape).contiguous()
-    # contiguous working vectors in the operator's cube layout
-    r = torch.empty(shape, dtype=b.dtype, device=dev)
-    q = torch.empty(shape, dtype=b.dtype, device=dev)
-    if x0 is None:
-        r.copy_(b.reshape(shape))
-    else:
-        A.apply_into(x, q)
-        r.copy_(b.reshape(shape) - q)
-    p = r.clone()
-    scal = torch.zeros(N.TG_CG_SCAL, dtype=torch.float64, device=dev)
-    lib = N.lib()
-    s = S.stream_handle()
-    N.check(lib.tg_cg_dot(code, n, r.data_ptr(), r.data_ptr(), scal.data_ptr(), 0, s))
-    N.check(lib.tg_cg_dot(code, n, r.data_ptr(), r.data_ptr(), scal.data_ptr(), 5, s))
-    it = 0
-    res = 1.0
-    for it in range(1, maxiter + 1):
-        A.apply_into(p, q)
-        N.check(lib.tg_cg_dot(code, n, p.data_ptr(), q.data_ptr(), scal.data_ptr(), 1, s))
-        N.check(lib.tg_cg_update(code, n, x.data_ptr(), r.data_ptr(), p.data_ptr(), q.data_ptr(),
-                                 scal.data_ptr(), s))
-        N.check(lib.tg_cg_direction(code, n, r.data_ptr(), p.data_ptr(), scal.data_ptr(), s))
-        if callback is not None:
-            callback(it, x)
-        if tol is not None and (it % check_every == 0 or it == maxiter):
-            v = scal[[0, 5]].cpu().numpy()
-            res = float(np.sqrt(v[0] / v[1])) if v[1] > 0 else 0.0
-            if res <= tol:
-                break
-    v = scal[[0, 5]].cpu().numpy()
-    res = float(np.sqrt(v[0] / v[1])) if v[1] > 0 else 0.0
-    return x, {"iterations": it, "residual": res}
-
-
-def reconstruct(P, y, lam=1e-2, identity=True, maxiter=50, tol=None, x0=None, axes=None,
-                check_every=10):
-    """Solves the regularised normal equations for data y (device, P.shapeout)."""
-    A = NormalOperator(P, lam=lam, identity=identity, axes=axes)
-    b = torch.zeros(tuple(P.shapein), dtype=P.tdtype, device=P.dev)
-    P.backproject_into(P._as_stack(y), b)
-    return cg(A, b, x0=x0, maxiter=maxiter, tol=tol, check_every=check_every)

@@ -1,0 +1,188 @@
+// Many uneven projector (or backprojector) jobs in one persistent launch.
+//
+// The host lays the jobs out in queue order (the schedule) in a device job
+// table; every job contributes its ray tiles (workgroup-sized pixel tiles of
+// its views) to one global tile range.  Persistent workgroups take the next
+// tile from an atomic counter until the range is drained, find the owning job
+// by binary search over the jobs' first-tile offsets, and run the same
+// per-ray body as the one-job kernels, so each job's results are those of a
+// one-job launch (bit-identical for the forward projector).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "siddon_common.h"
+#include "siddon_host.h"
+#include "siddon_rays.h"
+
+namespace tg {
+
+template <typename T>
+int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a);
+
+namespace {
+
+constexpr size_t kQueueHeader = 256;
+
+template <typename T>
+struct JobDesc {
+  KArgs<T> a;
+  unsigned long long tile_begin;
+  uint32_t sun;
+  uint32_t vox;
+};
+
+size_t job_bytes() {
+  return align_up(sizeof(JobDesc<double>) > sizeof(JobDesc<float>) ? sizeof(JobDesc<double>)
+                                                                    : sizeof(JobDesc<float>),
+                  16);
+}
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void run_tile(const KArgs<T> &a, uint32_t sun, uint32_t vox,
+                                         uint32_t tile) {
+  if (BWD) {
+    if (sun) {
+      if (vox) backproject_tile<T, true, true>(a, tile);
+      else backproject_tile<T, true, false>(a, tile);
+    } else {
+      if (vox) backproject_tile<T, false, true>(a, tile);
+      else backproject_tile<T, false, false>(a, tile);
+    }
+  } else {
+    if (sun) {
+      if (vox) project_tile<T, true, true>(a, tile);
+      else project_tile<T, true, false>(a, tile);
+    } else {
+      if (vox) project_tile<T, false, true>(a, tile);
+      else project_tile<T, false, false>(a, tile);
+    }
+  }
+}
+
+template <typename T, bool BWD>
+__global__ __launch_bounds__(256) void k_batch(const JobDesc<T> *__restrict__ jobs, int njobs,
+                                               unsigned long long total,
+                                               unsigned long long *head) {
+  __shared__ unsigned long long s_tile;
+  while (true) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(head, 1ull);
+    __syncthreads();
+    const unsigned long long tile = s_tile;
+    __syncthreads();
+    if (tile >= total) break;  // every workgroup reaches this once the range is drained
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].tile_begin <= tile) lo = mid;
+      else hi = mid - 1;
+    }
+    const JobDesc<T> &jd = jobs[lo];
+    run_tile<T, BWD>(jd.a, jd.sun, jd.vox, (uint32_t)(tile - jd.tile_begin));
+  }
+}
+
+__global__ void k_reset_head(unsigned long long *head) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *head = 0ull;
+}
+
+template <typename T>
+int run_batch(const tg_call *calls, int njobs, const tg_batch_plan *plan, void *queue,
+              void *host_staging, hipStream_t s) {
+  const int kind0 = calls[0].kind;
+  const bool bwd = (kind0 == TG_BPJ || kind0 == TG_BPJT);
+  std::vector<KArgs<T>> args(njobs);
+  std::vector<double> cost(njobs);
+  std::vector<unsigned long long> tiles(njobs);
+  for (int k = 0; k < njobs; k++) {
+    const tg_call &c = calls[k];
+    if (c.dtype != calls[0].dtype) return set_error(TG_EINVAL, "batch jobs must share a dtype");
+    const bool kb = (c.kind == TG_BPJ || c.kind == TG_BPJT);
+    if (kb != bwd) return set_error(TG_EINVAL, "batch jobs must all project or all backproject");
+    if (c.flags & TG_DETERMINISTIC)
+      return set_error(TG_EINVAL, "deterministic backprojection is not batched");
+    if (!c.cube || !c.data) return set_error(TG_EINVAL, "job %d: cube and data are required", k);
+    tg_call cc = c;
+    if (plan && plan->tile_j > 0) cc.tile_j = plan->tile_j;
+    int rc = make_args<T>(&cc, 0, cc.nv, args[k]);
+    if (rc) {
+      const std::string msg = tg_last_error();
+      return set_error(rc, "job %d: %s", k, msg.c_str());
+    }
+    tiles[k] = (unsigned long long)cc.nv * args[k].tiles_i * args[k].tiles_j;
+    // longest rays first: a ray crosses at most n0 + n1 + n2 voxels
+    cost[k] = (double)(cc.map.n[0] + cc.map.n[1] + cc.map.n[2]);
+  }
+  std::vector<int> order(njobs);
+  std::iota(order.begin(), order.end(), 0);
+  if (plan && plan->schedule == 1)
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  char *stage = (char *)host_staging;
+  memset(stage, 0, kQueueHeader);
+  unsigned long long begin = 0;
+  const size_t jb = job_bytes();
+  for (int q = 0; q < njobs; q++) {
+    const int k = order[q];
+    JobDesc<T> jd;
+    memset(&jd, 0, sizeof(jd));
+    jd.a = args[k];
+    jd.tile_begin = begin;
+    jd.sun = (calls[k].flags & TG_OBSTACLE_SUN) ? 1u : 0u;
+    jd.vox = (calls[k].voxmask != nullptr || (!bwd && calls[k].vox_weight != nullptr)) ? 1u : 0u;
+    memcpy(stage + kQueueHeader + (size_t)q * jb, &jd, sizeof(jd));
+    begin += tiles[k];
+  }
+  const unsigned long long total = begin;
+  const size_t bytes = kQueueHeader + (size_t)njobs * jb;
+  int rc;
+  if ((rc = hip_check(hipMemcpyAsync(queue, stage, bytes, hipMemcpyHostToDevice, s),
+                      "batch job table copy")))
+    return rc;
+  if (total == 0) return TG_OK;
+  unsigned long long *head = (unsigned long long *)queue;
+  const JobDesc<T> *jobs = (const JobDesc<T> *)((char *)queue + kQueueHeader);
+  int blocks = plan ? plan->blocks : 0;
+  if (blocks <= 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    blocks = cus * 8;
+  }
+  if ((unsigned long long)blocks > total) blocks = (int)total;
+  hipLaunchKernelGGL(k_reset_head, dim3(1), dim3(64), 0, s, head);
+  if (bwd)
+    hipLaunchKernelGGL((k_batch<T, true>), dim3(blocks), dim3(256), 0, s, jobs, njobs, total, head);
+  else
+    hipLaunchKernelGGL((k_batch<T, false>), dim3(blocks), dim3(256), 0, s, jobs, njobs, total, head);
+  return hip_check(hipGetLastError(), "batch kernel launch");
+}
+
+}  // namespace
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" {
+
+size_t tg_batch_queue_bytes(int32_t njobs) {
+  if (njobs < 0) return 0;
+  return kQueueHeader + (size_t)njobs * job_bytes();
+}
+
+int tg_batch_run(const tg_call *calls, int32_t njobs, const tg_batch_plan *plan, void *queue,
+                 void *host_staging, void *stream) {
+  if (njobs <= 0 || !calls) return set_error(TG_EINVAL, "empty batch");
+  if (!queue || !host_staging) return set_error(TG_EINVAL, "batch needs queue and staging buffers");
+  hipStream_t s = (hipStream_t)stream;
+  if (calls[0].dtype == TG_F32) return run_batch<float>(calls, njobs, plan, queue, host_staging, s);
+  if (calls[0].dtype == TG_F64) return run_batch<double>(calls, njobs, plan, queue, host_staging, s);
+  return set_error(TG_EINVAL, "bad dtype %d", calls[0].dtype);
+}
+
+}  // extern "C"

@@ -118,6 +118,147 @@ __global__ void k_cg_shift(double *scal) {
   }
 }
 
+// ---- segmented (multi-request) CG kernels --------------------------------
+// One launch serves every request of a batch: block b belongs to the segment
+// whose block0 <= b < block0 + blocks, found by binary search (uniform per
+// block).  Each block covers kSegElems consecutive elements of its segment.
+constexpr int kSegElems = 1024;
+
+__device__ __forceinline__ int find_segment(const tg_cg_seg *segs, int nseg, int64_t b) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T>
+__device__ __forceinline__ T reg_value(const tg_cg_seg &g, const T *x, int64_t k) {
+  const int64_t n0 = g.shape[0], n1 = g.shape[1], n2 = g.shape[2], n3 = g.shape[3];
+  const int64_t s3 = 1, s2 = n3, s1 = n2 * n3, s0 = n1 * n2 * n3;
+  const int64_t m[4] = {k / s0, (k / s1) % n1, (k / s2) % n2, k % n3};
+  const int64_t n[4] = {n0, n1, n2, n3};
+  const int64_t st[4] = {s0, s1, s2, s3};
+  const T xc = x[k];
+  T acc = g.identity ? xc : (T)0;
+#pragma unroll
+  for (int ax = 0; ax < 4; ax++) {
+    if (!(g.axes & (1u << ax))) continue;
+    if (m[ax] >= 1) acc += xc - x[k - st[ax]];
+    if (m[ax] <= n[ax] - 2) acc -= x[k + st[ax]] - xc;
+  }
+  return acc;
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void k_cg_seg(const tg_cg_seg *__restrict__ segs, int nseg,
+                                                double *partials) {
+  __shared__ double lds[256];
+  const int sidx = find_segment(segs, nseg, blockIdx.x);
+  const tg_cg_seg &g = segs[sidx];
+  const int64_t base = ((int64_t)blockIdx.x - g.block0) * kSegElems;
+  T *x = (T *)g.x;
+  T *r = (T *)g.r;
+  T *p = (T *)g.p;
+  T *q = (T *)g.q;
+  double acc = 0.0;
+  T coef = 0;
+  if (OP == TG_CG_OP_UPDATE) {
+    const double den = g.scal[1];
+    coef = (T)(den != 0.0 ? g.scal[0] / den : 0.0);
+  } else if (OP == TG_CG_OP_DIRECTION) {
+    const double den = g.scal[0];
+    coef = (T)(den != 0.0 ? g.scal[2] / den : 0.0);
+  }
+  for (int it = 0; it < kSegElems / 256; it++) {
+    const int64_t k = base + it * 256 + threadIdx.x;
+    if (k >= g.n) break;
+    if (OP == TG_CG_OP_REGULARIZE) {
+      if (!(g.voxmask && g.voxmask[k])) q[k] = q[k] + (T)g.lam * reg_value<T>(g, p, k);
+    } else if (OP == TG_CG_OP_DOT_PQ) {
+      acc += (double)p[k] * (double)q[k];
+    } else if (OP == TG_CG_OP_UPDATE) {
+      x[k] = x[k] + coef * p[k];
+      const T rk = r[k] - coef * q[k];
+      r[k] = rk;
+      acc += (double)rk * (double)rk;
+    } else if (OP == TG_CG_OP_DIRECTION) {
+      p[k] = r[k] + coef * p[k];
+    } else if (OP == TG_CG_OP_INIT) {
+      acc += (double)r[k] * (double)r[k];
+    } else if (OP == TG_CG_OP_ZERO_Q) {
+      q[k] = (T)0;
+    }
+  }
+  if (OP == TG_CG_OP_DOT_PQ || OP == TG_CG_OP_UPDATE || OP == TG_CG_OP_INIT) {
+    lds[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = lds[0];
+  }
+}
+
+// One workgroup per segment: reduce its partials in a fixed order and update
+// the segment's scalars.
+template <int OP>
+__global__ __launch_bounds__(256) void k_cg_seg_final(const tg_cg_seg *__restrict__ segs,
+                                                      const double *partials) {
+  __shared__ double lds[256];
+  const tg_cg_seg &g = segs[blockIdx.x];
+  const int64_t nb = (g.n + kSegElems - 1) / kSegElems;
+  double s = 0.0;
+  if (OP != TG_CG_OP_DIRECTION)
+    for (int64_t k = threadIdx.x; k < nb; k += 256) s += partials[g.block0 + k];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double *sc = g.scal;
+  if (OP == TG_CG_OP_DOT_PQ) {
+    sc[1] = lds[0];
+  } else if (OP == TG_CG_OP_UPDATE) {
+    sc[3] = sc[1] != 0.0 ? sc[0] / sc[1] : 0.0;
+    sc[2] = lds[0];
+  } else if (OP == TG_CG_OP_DIRECTION) {
+    sc[4] = sc[0] != 0.0 ? sc[2] / sc[0] : 0.0;
+    sc[0] = sc[2];
+  } else if (OP == TG_CG_OP_INIT) {
+    sc[0] = lds[0];
+    sc[5] = lds[0];
+  }
+}
+
+template <typename T, int OP>
+int launch_seg(const tg_cg_seg *segs, int nseg, int64_t nblocks, double *partials, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_seg<T, OP>), dim3((uint32_t)nblocks), dim3(256), 0, s, segs, nseg,
+                     partials);
+  if (OP != TG_CG_OP_REGULARIZE && OP != TG_CG_OP_ZERO_Q)
+    hipLaunchKernelGGL((k_cg_seg_final<OP>), dim3((uint32_t)nseg), dim3(256), 0, s, segs, partials);
+  return hip_check(hipGetLastError(), "segmented cg launch");
+}
+
+template <typename T>
+int dispatch_seg(int op, const tg_cg_seg *segs, int nseg, int64_t nblocks, double *partials,
+                 hipStream_t s) {
+  switch (op) {
+    case TG_CG_OP_REGULARIZE: return launch_seg<T, TG_CG_OP_REGULARIZE>(segs, nseg, nblocks, partials, s);
+    case TG_CG_OP_DOT_PQ: return launch_seg<T, TG_CG_OP_DOT_PQ>(segs, nseg, nblocks, partials, s);
+    case TG_CG_OP_UPDATE: return launch_seg<T, TG_CG_OP_UPDATE>(segs, nseg, nblocks, partials, s);
+    case TG_CG_OP_DIRECTION: return launch_seg<T, TG_CG_OP_DIRECTION>(segs, nseg, nblocks, partials, s);
+    case TG_CG_OP_INIT: return launch_seg<T, TG_CG_OP_INIT>(segs, nseg, nblocks, partials, s);
+    case TG_CG_OP_ZERO_Q: return launch_seg<T, TG_CG_OP_ZERO_Q>(segs, nseg, nblocks, partials, s);
+  }
+  return set_error(TG_EINVAL, "bad segmented cg op %d", op);
+}
+
 inline uint32_t red_blocks(int64_t n) {
   int64_t b = (n + 255) / 256;
   if (b > kRedBlocks) b = kRedBlocks;
@@ -207,6 +348,18 @@ int tg_cg_direction(int32_t dtype, int64_t n, const void *r, void *p, double *sc
     return set_error(TG_EINVAL, "bad dtype %d", dtype);
   hipLaunchKernelGGL(k_cg_shift, dim3(1), dim3(64), 0, s, scal);
   return hip_check(hipGetLastError(), "cg direction launch");
+}
+
+int64_t tg_cg_seg_blocks(int64_t n) { return (n + kSegElems - 1) / kSegElems; }
+
+int tg_cg_batch(int32_t dtype, int32_t op, const tg_cg_seg *segs, int32_t nseg, int64_t nblocks,
+                double *partials, void *stream) {
+  if (!segs || nseg <= 0 || nblocks <= 0 || !partials)
+    return set_error(TG_EINVAL, "bad tg_cg_batch args");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == TG_F32) return dispatch_seg<float>(op, segs, nseg, nblocks, partials, s);
+  if (dtype == TG_F64) return dispatch_seg<double>(op, segs, nseg, nblocks, partials, s);
+  return set_error(TG_EINVAL, "bad dtype %d", dtype);
 }
 
 }  // extern "C"

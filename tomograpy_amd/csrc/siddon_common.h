@@ -55,6 +55,7 @@ struct KArgs {
   int32_t bj_log2;       // log2(pixels per block row)
   int32_t tiles_i, tiles_j;  // tiles per view
   int32_t kind;
+  uint32_t flags;  // TG_ZERO_OUT
   int64_t ds[3];  // data strides
   int64_t ms[3];  // pixel-mask strides
   int64_t vs[3];  // voxel mask / voxel weight strides

@@ -1,0 +1,231 @@
+"""Continuous-batching reconstruction server.
+
+Requests (an image stack with its headers, and CG settings) arrive over time.
+The server admits a waiting request as soon as its HBM footprint fits the
+budget and a slot is free, keeps its cube, images and CG state on the device,
+and advances every admitted request by one CG iteration per engine step: one
+batched projector launch and one batched backprojector launch over all of
+them (``tg_batch_run``), plus the segmented CG vector kernels
+(``tg_cg_batch``).  A request leaves when its relative residual reaches its
+tolerance (checked every ``check_every`` of its iterations) or at maxiter;
+requests that arrive meanwhile join at the next engine step.
+
+``serve_plain`` is the per-request baseline: the same requests solved one at
+a time, in arrival order, by ``cg.solve`` (the same kernels, one job at a
+time).  Both report p50/p99 latency (completion - arrival) and throughput.
+"""
+import collections
+import time
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import cg as CG
+from .batch import Autotuner
+from .lo_wrapper import Siddon, Siddon4d
+
+
+class Request:
+    """One reconstruction: headers, image stack (host), solver settings."""
+
+    def __init__(self, rid, cube_header, data_header, y, arrival=0.0, lam=1e-2, tol=1e-4,
+                 maxiter=200, mask=None, tindex=None, obstacle=None, dtype=np.float32):
+        self.id = rid
+        self.cube_header = cube_header
+        self.data_header = data_header
+        self.y = y
+        self.arrival = float(arrival)
+        self.lam = lam
+        self.tol = tol
+        self.maxiter = maxiter
+        self.mask = mask
+        self.tindex = tindex
+        self.obstacle = obstacle
+        self.dtype = np.dtype(dtype)
+        self.x = None
+        self.iterations = 0
+        self.residual = None
+        self.t_admit = None
+        self.t_done = None
+
+    def footprint(self):
+        """Device bytes while admitted: 4 CG vectors + 2 operator buffers + images."""
+        n = np.prod([self.cube_header["NAXIS%d" % (k + 1)] for k in range(3)])
+        nt = 1 if self.tindex is None else int(np.max(self.tindex)) + 1
+        n1 = self.data_header[0]["NAXIS1"]
+        n2 = self.data_header[0]["NAXIS2"]
+        pix = n1 * n2 * len(self.data_header)
+        b = self.dtype.itemsize
+        trig = len(self.data_header) * (n1 + n2) * 16
+        return int((6 * n * nt + 2 * pix) * b + trig + (pix if self.mask is not None else 0) + (1 << 16))
+
+    def operator(self):
+        if self.tindex is not None:
+            return Siddon4d(self.data_header, self.cube_header, tindex=self.tindex, mask=self.mask,
+                            obstacle=self.obstacle, dtype=self.dtype)
+        return Siddon(self.data_header, self.cube_header, mask=self.mask, obstacle=self.obstacle,
+                      dtype=self.dtype)
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.arrival
+
+
+def make_requests(mix, rng, rate=None, tol=1e-4, lam=1e-2, maxiter=200, dtype=np.float32,
+                  noise=0.01):
+    """Requests of the served stream from a size mix (workloads.config5_mix).
+
+    Each request images a seeded blob corona (time bins scale it by 1 + 0.05 t)
+    from random observers at the mix's distance; y = P x + noise * max * N(0, 1).
+    Arrival times are a Poisson process of the given rate (all at 0 if None).
+    """
+    from . import models as Mo
+    from . import workloads as W
+    reqs = []
+    t = 0.0
+    for k, spec in enumerate(mix):
+        ch = W.cube_header(spec["n"], dtype=dtype)
+        lons = rng.uniform(0, 2 * np.pi, spec["nv"])
+        lats = np.deg2rad(rng.uniform(-7.0, 7.0, spec["nv"]))
+        hs = W.view_headers(spec["npix"], lons, lats, d=spec["d"], dtype=dtype)
+        x = W.blob_phantom_torch(ch, 8, rng)
+        tindex = None
+        if spec["bins"] > 1:
+            tindex = (np.arange(spec["nv"]) % spec["bins"]).astype(np.int32)
+            x = torch.stack([x * (1 + 0.05 * b) for b in range(spec["bins"])], dim=-1)
+        mask = None
+        if spec["mask"]:
+            mask = Mo.radius_data_mask(hs, spec["npix"], spec["npix"], data_rmin=1.05)
+        r = Request(k, ch, hs, None, lam=lam, tol=tol, maxiter=maxiter, mask=mask, tindex=tindex,
+                    dtype=dtype)
+        P = r.operator()
+        y = P.matvec(x.to(P.dev, P.tdtype)).clone()
+        nz = torch.from_numpy(rng.standard_normal(tuple(y.shape))).to(y.device, y.dtype)
+        y += noise * float(y.abs().max()) * nz
+        r.y = y.cpu().numpy()
+        if rate:
+            t += float(rng.exponential(1.0 / rate))
+        r.arrival = t
+        reqs.append(r)
+    return reqs
+
+
+def _y_device(req, P):
+    y = torch.as_tensor(np.asarray(req.y))
+    buf = torch.empty((P.nv, P.n1, P.n2), dtype=P.tdtype, device=P.dev).permute(1, 2, 0)
+    buf.copy_(y.to(P.dev, P.tdtype), non_blocking=False)
+    return buf
+
+
+def _finish(req, s, t0):
+    req.x = s.x.cpu().numpy()
+    req.iterations = s.iterations
+    req.residual = s.residual()
+    req.t_done = time.perf_counter() - t0
+
+
+def summarize(requests, t_first, t_end):
+    lat = np.array([r.latency for r in requests])
+    span = t_end - t_first
+    return {"requests": len(requests), "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
+            "makespan_s": span, "throughput_rps": len(requests) / span if span > 0 else float("inf"),
+            "iterations": int(sum(r.iterations for r in requests))}
+
+
+class Server:
+    """Continuous batching over CG reconstructions on one GPU."""
+
+    def __init__(self, budget_bytes=64 << 30, max_active=64, check_every=5, tuner=None):
+        self.budget = int(budget_bytes)
+        self.max_active = int(max_active)
+        self.check_every = int(check_every)
+        # the active set changes every few steps: a fixed plan unless a tuner is given
+        self.tuner = tuner
+        self.dev = torch.device("cuda", torch.cuda.current_device())
+        self.scal_pool = torch.zeros((self.max_active, 16), dtype=torch.float64, device=self.dev)
+        self.free_rows = list(range(self.max_active))
+
+    def run(self, requests):
+        """Serves `requests` (arrival times in seconds from now); returns a summary."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = collections.deque()
+        active = []  # (req, solve, row)
+        used = 0
+        table = None
+        t0 = time.perf_counter()
+        steps = 0
+        while pending or waiting or active:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            admitted = []
+            while waiting and len(active) < self.max_active:
+                req = waiting[0]
+                fp = req.footprint()
+                if fp > self.budget:
+                    raise ValueError("request %s needs %d bytes, budget %d" % (req.id, fp, self.budget))
+                if used + fp > self.budget:
+                    break
+                waiting.popleft()
+                used += fp
+                P = req.operator()
+                row = self.free_rows.pop()
+                s = CG.CGSolve(P, _y_device(req, P), lam=req.lam, tol=req.tol, maxiter=req.maxiter,
+                               scal=self.scal_pool[row])
+                req.t_admit = time.perf_counter() - t0
+                active.append((req, s, row))
+                admitted.append(s)
+            if admitted:
+                CG.SegTable(admitted, self.dev).op(N.TG_CG_OP_INIT)
+                table = None
+            if not active:
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
+                continue
+            if table is None:
+                table = CG.SegTable([s for _, s, _ in active], self.dev)
+            CG.iterate([s for _, s, _ in active], table, self.tuner if len(active) > 1 else None)
+            steps += 1
+            due = [k for k, (r, s, _) in enumerate(active)
+                   if s.iterations % self.check_every == 0 or s.iterations >= r.maxiter]
+            if due:
+                scal = self.scal_pool.cpu().numpy()
+                keep = []
+                for k, (r, s, row) in enumerate(active):
+                    if k in due:
+                        res = s.residual(scal[row])
+                        if res <= r.tol or s.iterations >= r.maxiter:
+                            _finish(r, s, t0)
+                            used -= r.footprint()
+                            self.free_rows.append(row)
+                            continue
+                    keep.append((r, s, row))
+                if len(keep) != len(active):
+                    active = keep
+                    table = None
+        t_end = time.perf_counter() - t0
+        out = summarize(requests, min(r.arrival for r in requests), t_end)
+        out["engine_steps"] = steps
+        return out
+
+
+def serve_plain(requests, check_every=5):
+    """Per-request baseline: solve each request alone, in arrival order."""
+    order = sorted(requests, key=lambda r: r.arrival)
+    t0 = time.perf_counter()
+    for req in order:
+        now = time.perf_counter() - t0
+        if req.arrival > now:
+            time.sleep(req.arrival - now)
+        req.t_admit = time.perf_counter() - t0
+        P = req.operator()
+        x, info = CG.solve(P, _y_device(req, P), lam=req.lam, maxiter=req.maxiter, tol=req.tol,
+                           check_every=check_every)
+        req.x = x.cpu().numpy()
+        req.iterations = info["iterations"]
+        req.residual = info["residual"]
+        req.t_done = time.perf_counter() - t0
+    t_end = time.perf_counter() - t0
+    return summarize(requests, min(r.arrival for r in requests), t_end)

@@ -81,6 +81,32 @@ def blob_phantom(header, n_blobs, rng, shell=(1.05, 1.4), sigma=(0.05, 0.2), amp
     return out.astype(dtype)
 
 
+def blob_phantom_torch(header, n_blobs, rng, shell=(1.05, 1.4), sigma=(0.05, 0.2),
+                       amp=(0.5, 2.0), r_zero=1.0, device="cuda"):
+    """blob_phantom evaluated with torch on the device (same draws from rng), float64."""
+    import torch
+    axes = []
+    for k in range(3):
+        n = int(header["NAXIS%d" % (k + 1)])
+        cd = header["CDELT%d" % (k + 1)]
+        mmin = -header["CRPIX%d" % (k + 1)] * cd
+        axes.append(torch.as_tensor(mmin + (np.arange(n) + 0.5) * cd, device=device))
+    x, y, z = torch.meshgrid(*axes, indexing="ij")
+    out = torch.zeros(x.shape, dtype=torch.float64, device=device)
+    r3 = rng.uniform(shell[0] ** 3, shell[1] ** 3, n_blobs)
+    rad = np.cbrt(r3)
+    v = rng.normal(size=(n_blobs, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    centres = v * rad[:, None]
+    sig = rng.uniform(sigma[0], sigma[1], n_blobs)
+    amps = rng.uniform(amp[0], amp[1], n_blobs)
+    for c, s, a in zip(centres, sig, amps):
+        d2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
+        out += a * torch.exp(-0.5 * d2 / (s * s))
+    out[torch.sqrt(x * x + y * y + z * z) < r_zero] = 0.0
+    return out
+
+
 def config1(dtype=np.float32):
     """(cube_header, cube, view_headers) of C1."""
     rng = rng_for(1)
@@ -108,6 +134,21 @@ def config3_geometry(dtype=np.float32, n=128, n_views=64, n_pix=256, n_bins=8):
     tindex = (np.arange(n_views) % n_bins).astype(np.int32)
     voxmask = voxel_radius(ch) < 1.05
     return ch, hs, tindex, voxmask, rng
+
+
+def config5_mix(n_requests, rng, dtype=np.float32, n_range=(32, 256), v_range=(8, 128),
+                pix=(64, 128, 256, 512), max_bins=8):
+    """Size mix of the served stream: list of dicts with the cube edge, view
+    count, image size, mask flag and time-bin count of each request."""
+    out = []
+    for _ in range(n_requests):
+        out.append({"n": int(rng.integers(n_range[0], n_range[1] + 1)),
+                    "nv": int(rng.integers(v_range[0], v_range[1] + 1)),
+                    "npix": int(rng.choice(pix)),
+                    "mask": bool(rng.integers(0, 2)),
+                    "bins": int(rng.integers(1, max_bins + 1)),
+                    "d": float(rng.uniform(150.0, 250.0))})
+    return out
 
 
 def config4_jobs(n_jobs=64, dtype=np.float32, max_n=192, max_views=96, pix=(64, 128, 256, 512)):

@@ -226,6 +226,17 @@ size_t tg_batch_queue_bytes(int32_t njobs);
 int tg_batch_run(const tg_call *calls, int32_t njobs, const tg_batch_plan *plan, void *queue,
                  void *host_staging, void *stream);
 
+/* Host-only: writes the job table tg_batch_run would upload into host_staging
+ * and launches nothing (for inspection and tests).  Header words: [1] = total
+ * tiles, [2] = njobs; jobs follow at byte 256 in queue order. */
+int tg_batch_plan_table(const tg_call *calls, int32_t njobs, const tg_batch_plan *plan,
+                        void *host_staging);
+
+/* Layout of one job-table entry for `dtype`: its size, the byte offset of its
+ * first-tile index (uint64) and of its cube pointer. */
+int tg_batch_job_layout(int32_t dtype, size_t *job_bytes, size_t *tile_begin_offset,
+                        size_t *cube_offset);
+
 /* Thread-local message describing the last error on this thread. */
 const char *tg_last_error(void);
 

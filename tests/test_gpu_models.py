@@ -107,11 +107,22 @@ def test_cg_matches_scipy_fp64(oracle):
 
     b = np.zeros((12, 12, 12))
     oracle.siddon(oracle.BPJ, recs, mrec, b, np.ascontiguousarray(y), sun=True, nthread=1)
-    its = 8
-    xs, _ = scipy_cg(LinearOperator((b.size, b.size), matvec=A), b.ravel(), maxiter=its, rtol=1e-30, atol=0)
-    xg, info = CG.reconstruct(P, torch.from_numpy(y).cuda(), lam=lam, maxiter=its)
-    assert info["iterations"] == its
-    assert rel_l2(xg.cpu().numpy().ravel(), xs) < 1e-9
+    # the operator itself first: one application against the oracle's
+    A_gpu = CG.NormalOperator(P, lam=lam, identity=True)
+    v = rng.uniform(size=(12, 12, 12))
+    av = A_gpu.apply_into(torch.from_numpy(v).cuda(), torch.zeros((12, 12, 12), dtype=torch.float64,
+                                                                    device="cuda"))
+    err_a = rel_l2(av.cpu().numpy().ravel(), A(v.ravel()))
+    print("operator rel err", err_a, flush=True)
+    assert err_a < 1e-13
+    for its in (1, 2, 8):
+        op = LinearOperator((b.size, b.size), matvec=A, dtype=np.float64)
+        xs, _ = scipy_cg(op, b.ravel(), maxiter=its, rtol=1e-30, atol=0)
+        xg, info = CG.reconstruct(P, torch.from_numpy(y).cuda(), lam=lam, maxiter=its)
+        err = rel_l2(xg.cpu().numpy().ravel(), xs)
+        print("cg iterations", its, "rel err", err, flush=True)
+        assert info["iterations"] == its
+        assert err < 1e-9, (its, err)
 
 
 def test_cg_converges_to_tolerance():

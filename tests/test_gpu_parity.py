@@ -115,11 +115,13 @@ def test_deterministic_small_workspace_chunks_views(oracle):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_step_counts_match_oracle(oracle, dtype):
     ch, cube, data = c1_problem(dtype, n_views=8)
-    ref = oracle.ray_steps(map_record(ch), cube.shape, view_records(data.header), 128, 128)
+    ref = oracle.ray_steps(map_record(ch), cube.shape, view_records(data.header), 128, 128,
+                           dtype=dtype)
     total, per_ray = T.count_steps(data, cube, per_ray=True)
     assert np.array_equal(per_ray.cpu().numpy().astype(np.uint32), ref)
     assert total == int(ref.sum())
-    ref_sun = oracle.ray_steps(map_record(ch), cube.shape, view_records(data.header), 128, 128, sun=True)
+    ref_sun = oracle.ray_steps(map_record(ch), cube.shape, view_records(data.header), 128, 128,
+                               sun=True, dtype=dtype)
     assert T.count_steps(data, cube, obstacle="sun") == int(ref_sun.sum())
 
 

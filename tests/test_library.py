@@ -113,3 +113,53 @@ def test_call_validation_errors_without_gpu():
     assert N.lib().tg_conic_full_projector(ctypes.byref(call), None) == N.TG_EINVAL
     call.tile_j = 16
     assert N.lib().tg_conic_image_projector(ctypes.byref(call), 5, None) == N.TG_EINVAL
+
+
+@pytest.mark.parametrize("dtype_code", [N.TG_F32, N.TG_F64])
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_batch_job_table_layout(dtype_code, schedule):
+    """The staged job table has the stride and fields the persistent kernel reads
+    (entries JobDesc<T>[k] in queue order, first-tile prefix, per-job pointers)."""
+    rng = np.random.default_rng(0)
+    njobs = 7
+    calls = (N.TgCall * njobs)()
+    expect_tiles = []
+    sizes = []
+    for k in range(njobs):
+        c = calls[k]
+        c.dtype = dtype_code
+        c.kind = N.TG_PJ
+        n = int(rng.integers(8, 40))
+        c.map.n[0] = c.map.n[1] = c.map.n[2] = n
+        c.map.nt = 1
+        c.n1 = c.n2 = int(rng.choice([16, 24, 40]))
+        c.nv = int(rng.integers(1, 6))
+        c.views = 0x1000 + k
+        c.cube = 0x100000 * (k + 1)
+        c.data = 0x200000 * (k + 1)
+        sizes.append(3 * n)
+        tj = 16
+        bi = 256 // tj
+        expect_tiles.append(c.nv * ((c.n1 + bi - 1) // bi) * ((c.n2 + tj - 1) // tj))
+    plan = N.TgBatchPlan()
+    plan.tile_j = 16
+    plan.schedule = schedule
+    nbytes = N.lib().tg_batch_queue_bytes(njobs)
+    stage = np.zeros(nbytes, dtype=np.uint8)
+    N.check(N.lib().tg_batch_plan_table(calls, njobs, ctypes.byref(plan), stage.ctypes.data))
+    jb, to, co = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+    N.check(N.lib().tg_batch_job_layout(dtype_code, ctypes.byref(jb), ctypes.byref(to), ctypes.byref(co)))
+    assert 256 + njobs * jb.value <= nbytes
+    order = list(range(njobs))
+    if schedule == 1:
+        order = sorted(order, key=lambda k: -sizes[k])  # stable: longest rays first
+    hdr = stage[:256].view(np.uint64)
+    assert hdr[1] == sum(expect_tiles) and hdr[2] == njobs
+    begin = 0
+    for q, k in enumerate(order):
+        base = 256 + q * jb.value
+        tb = int(stage[base + to.value: base + to.value + 8].view(np.uint64)[0])
+        cube = int(stage[base + co.value: base + co.value + 8].view(np.uint64)[0])
+        assert tb == begin, (q, k)
+        assert cube == calls[k].cube, (q, k)
+        begin += expect_tiles[k]

@@ -81,6 +81,10 @@ SIGNATURES = [
     ("tg_cg_seg_blocks", _i64, [_i64]),
     ("tg_cg_batch", ctypes.c_int, [_i32, _i32, _vp, _i32, _i64, _vp, _vp]),
     ("tg_batch_queue_bytes", _sz, [_i32]),
+    ("tg_batch_plan_table", ctypes.c_int, [ctypes.POINTER(TgCall), _i32,
+                                           ctypes.POINTER(TgBatchPlan), _vp]),
+    ("tg_batch_job_layout", ctypes.c_int, [_i32, ctypes.POINTER(_sz), ctypes.POINTER(_sz),
+                                           ctypes.POINTER(_sz)]),
     ("tg_batch_run", ctypes.c_int, [ctypes.POINTER(TgCall), _i32, ctypes.POINTER(TgBatchPlan),
                                     _vp, _vp, _vp]),
 ]

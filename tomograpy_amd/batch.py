@@ -64,6 +64,7 @@ def run(ops, ins, outs, backward=False, plan=None, stream=None, zero_out=False):
         return outs
     calls = (N.TgCall * n)()
     post = []
+    copies = []
     for k, (P, x, y) in enumerate(zip(ops, ins, outs)):
         if backward:
             target = y
@@ -73,7 +74,12 @@ def run(ops, ins, outs, backward=False, plan=None, stream=None, zero_out=False):
             data = P._as_stack(x)
             c = P._call(N.TG_BPJ, target, data)
         else:
-            c = P._call(N.TG_PJ, P._as_cube(x), y)
+            target = y
+            if P.pix_weight is not None and y.stride() != P.pix_weight.stride():
+                # pixel weights live in the operator's view-major layout
+                target = P._as_stack(y)
+                copies.append((target, y))
+            c = P._call(N.TG_PJ, P._as_cube(x), target)
             if zero_out:
                 c.flags |= N.TG_ZERO_OUT
         ctypes.memmove(ctypes.byref(calls[k]), ctypes.byref(c), ctypes.sizeof(N.TgCall))
@@ -84,6 +90,8 @@ def run(ops, ins, outs, backward=False, plan=None, stream=None, zero_out=False):
                                  bufs.stage.data_ptr(), s))
     # the staging buffer is reused by the next launch: wait until the copy left it
     torch.cuda.current_stream().synchronize()
+    for tmp, y in copies:
+        y.copy_(tmp)
     for P, tmp, y in post:
         sc = S.make_call(N.TG_BPJ, P.code, P.xout.shape, P.xout.stride(), tmp.shape, tmp.stride(),
                          P.map_rec, P.views, tmp.data_ptr(), P.xout.data_ptr(),

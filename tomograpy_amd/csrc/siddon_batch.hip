@@ -8,6 +8,7 @@
 // per-ray body as the one-job kernels, so each job's results are those of a
 // one-job launch (bit-identical for the forward projector).
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
@@ -123,8 +124,9 @@ int run_batch(const tg_call *calls, int njobs, const tg_batch_plan *plan, void *
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
   char *stage = (char *)host_staging;
   memset(stage, 0, kQueueHeader);
+  // The device reads the table as JobDesc<T>[njobs]: pack it with that stride.
+  JobDesc<T> *table = (JobDesc<T> *)(stage + kQueueHeader);
   unsigned long long begin = 0;
-  const size_t jb = job_bytes();
   for (int q = 0; q < njobs; q++) {
     const int k = order[q];
     JobDesc<T> jd;
@@ -133,11 +135,15 @@ int run_batch(const tg_call *calls, int njobs, const tg_batch_plan *plan, void *
     jd.tile_begin = begin;
     jd.sun = (calls[k].flags & TG_OBSTACLE_SUN) ? 1u : 0u;
     jd.vox = (calls[k].voxmask != nullptr || (!bwd && calls[k].vox_weight != nullptr)) ? 1u : 0u;
-    memcpy(stage + kQueueHeader + (size_t)q * jb, &jd, sizeof(jd));
+    memcpy(&table[q], &jd, sizeof(jd));
     begin += tiles[k];
   }
   const unsigned long long total = begin;
-  const size_t bytes = kQueueHeader + (size_t)njobs * jb;
+  const size_t bytes = kQueueHeader + (size_t)njobs * sizeof(JobDesc<T>);
+  // the header records the job count and the tile total (read by tg_batch_plan_table)
+  ((unsigned long long *)stage)[1] = total;
+  ((unsigned long long *)stage)[2] = (unsigned long long)njobs;
+  if (!queue) return TG_OK;  // plan only: the table is in host_staging
   int rc;
   if ((rc = hip_check(hipMemcpyAsync(queue, stage, bytes, hipMemcpyHostToDevice, s),
                       "batch job table copy")))
@@ -183,6 +189,32 @@ int tg_batch_run(const tg_call *calls, int32_t njobs, const tg_batch_plan *plan,
   if (calls[0].dtype == TG_F32) return run_batch<float>(calls, njobs, plan, queue, host_staging, s);
   if (calls[0].dtype == TG_F64) return run_batch<double>(calls, njobs, plan, queue, host_staging, s);
   return set_error(TG_EINVAL, "bad dtype %d", calls[0].dtype);
+}
+
+int tg_batch_plan_table(const tg_call *calls, int32_t njobs, const tg_batch_plan *plan,
+                        void *host_staging) {
+  if (njobs <= 0 || !calls) return set_error(TG_EINVAL, "empty batch");
+  if (!host_staging) return set_error(TG_EINVAL, "batch needs a staging buffer");
+  if (calls[0].dtype == TG_F32) return run_batch<float>(calls, njobs, plan, nullptr, host_staging, 0);
+  if (calls[0].dtype == TG_F64) return run_batch<double>(calls, njobs, plan, nullptr, host_staging, 0);
+  return set_error(TG_EINVAL, "bad dtype %d", calls[0].dtype);
+}
+
+int tg_batch_job_layout(int32_t dtype, size_t *job_bytes_out, size_t *tile_begin_offset,
+                        size_t *cube_offset) {
+  if (dtype == TG_F32) {
+    *job_bytes_out = sizeof(JobDesc<float>);
+    *tile_begin_offset = offsetof(JobDesc<float>, tile_begin);
+    *cube_offset = offsetof(KArgs<float>, cube);
+    return TG_OK;
+  }
+  if (dtype == TG_F64) {
+    *job_bytes_out = sizeof(JobDesc<double>);
+    *tile_begin_offset = offsetof(JobDesc<double>, tile_begin);
+    *cube_offset = offsetof(KArgs<double>, cube);
+    return TG_OK;
+  }
+  return set_error(TG_EINVAL, "bad dtype %d", dtype);
 }
 
 }  // extern "C"

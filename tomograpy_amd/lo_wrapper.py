@@ -147,11 +147,22 @@ class Siddon:
             y = y.to(self.dev)
         if y.dtype != self.tdtype:
             raise ValueError("operand dtype %s, operator dtype %s" % (y.dtype, self.tdtype))
-        return y.reshape(self.shapeout) if y.dim() != 3 else y
+        y = y.reshape(self.shapeout) if y.dim() != 3 else y
+        if self.pix_weight is not None and y.stride() != self.pix_weight.stride():
+            # pixel weights are stored in the operator's view-major layout
+            buf = stack_buffer(self.n1, self.n2, self.nv, self.tdtype, self.dev)
+            buf.copy_(y)
+            y = buf
+        return y
 
     def project_into(self, x, out):
         """out += P x (no zeroing)."""
         x = self._as_cube(x)
+        if self.pix_weight is not None and out.stride() != self.pix_weight.stride():
+            tmp = self._as_stack(out)
+            self.project_into(x, tmp)
+            out.copy_(tmp)
+            return out
         c = self._call(N.TG_PJ, x, out)
         N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
         return out

@@ -105,6 +105,14 @@ def main():
     R_t = P._rays_traced
     V = len(hs)
     torch.cuda.synchronize()
+    plan_info = None
+    if P.use_plan():
+        t0 = time.perf_counter()
+        plan = P.build_plan()
+        torch.cuda.synchronize()
+        plan_info = {"build_ms": (time.perf_counter() - t0) * 1e3, "records": int(plan.records),
+                     "bytes": int(P.plan_bytes()), "brick": list(plan.brick),
+                     "chunks": int(plan.chunks)}
 
     x = x_true.clone()
     stream = torch.cuda.current_stream()
@@ -188,7 +196,9 @@ def main():
         "config": {"workload": "C2 pair: %d^3 cube, %d views of %d^2, D=214, LAT U(-7,7) deg"
                                % (args.n, V, args.pix),
                    "siddon_steps_per_projection": S, "rays_traced": R_t,
-                   "ms_forward": t_fwd * 1e3, "ms_backward": t_bwd * 1e3},
+                   "ms_forward": t_fwd * 1e3, "ms_backward": t_bwd * 1e3,
+                   "backprojection": "brick plan" if plan_info else "atomic scatter",
+                   "plan": plan_info},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": B_dom},

@@ -141,6 +141,39 @@ size_t tg_det_workspace_min_bytes(const tg_call *c);
  * voxel-weight half of the Thomson operator's adjoint (models.py:223-235). */
 int tg_scale_cube(const tg_call *c, void *stream);
 
+/* ---- brick-privatised backprojection (no reference counterpart; replaces the
+ * atomic scatter of C:923-928 / C:939-944 for large cubes) ----
+ * A plan, built once per geometry (views, map, mask, obstacle, time bins),
+ * records each ray's traversal state at every brick it enters.  A
+ * backprojection then runs one workgroup per chunk of a brick's records,
+ * accumulating the brick in LDS and flushing it with one coalesced atomic per
+ * touched voxel.  Lengths and order of operations per step are those of the
+ * continuous traversal; only the summation order into a voxel differs (as
+ * in the atomic mode). */
+typedef struct tg_bpj_plan {
+  int32_t brick[3];      /* brick edge per axis, powers of two; 0,0,0 = default (128 KiB) */
+  int32_t chunk_records; /* records per workgroup chunk; 0 = default */
+  int32_t nb_axis[3];    /* bricks per axis (set by tg_bpj_plan_init) */
+  int32_t nbricks;       /* bricks x time bins (set by tg_bpj_plan_init) */
+  int64_t records;       /* set by tg_bpj_plan_count */
+  int64_t chunks;        /* set by tg_bpj_plan_count */
+  void *meta;            /* device, caller-owned, meta_bytes */
+  size_t meta_bytes;     /* required size, set by tg_bpj_plan_init */
+  void *rec;             /* device, caller-owned, rec_bytes */
+  size_t rec_bytes;      /* required size, set by tg_bpj_plan_count */
+} tg_bpj_plan;
+
+/* Fills the derived fields and meta_bytes for the geometry of *c (kind bpj or
+ * bpjt; cube edges <= 1024). */
+int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p);
+/* Counts records per brick into p->meta; synchronises the stream; sets
+ * records, chunks and rec_bytes. */
+int tg_bpj_plan_count(const tg_call *c, tg_bpj_plan *p, void *stream);
+/* Writes the records into p->rec (asynchronous). */
+int tg_bpj_plan_build(const tg_call *c, const tg_bpj_plan *p, void *stream);
+/* cube += P^T data through the plan (asynchronous). */
+int tg_bpj_plan_backproject(const tg_call *c, const tg_bpj_plan *p, void *stream);
+
 /* ---- device-resident conjugate gradient (lo.acg consumer, srt.py:225) ---- */
 
 /* out += lam * ((identity ? x : 0) + sum_k D_k^T D_k x) on a C-contiguous

@@ -276,3 +276,59 @@ def test_torch_inputs_accumulate_in_place():
     T.check_status()
     # second call adds the same projection again (in-place accumulation)
     assert torch.allclose(y - y1, y1 - y0, rtol=1e-5, atol=1e-5)
+
+
+# ---- brick-privatised backprojection (plan) ---------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("obstacle", [None, "sun"])
+def test_plan_backprojection_c1(oracle, dtype, obstacle):
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    print("plan bpj C1", np.dtype(dtype).name, obstacle, flush=True)
+    ch, cube, data = c1_problem(dtype)
+    rng = np.random.default_rng(21)
+    y = rng.uniform(-1, 1, data.shape).astype(dtype)
+    ref = np.zeros(cube.shape, dtype)
+    oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref,
+                  T.InfoArray(y, header=data.header), sun=obstacle == "sun", nthread=1)
+    P = siddon_lo(data.header, ch, dtype=dtype, obstacle=obstacle, bpj_mode="plan")
+    out = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
+    T.check_status()
+    assert P._plan.records > 0
+    assert rel_l2(out, ref) <= (1e-5 if dtype == np.float32 else 1e-12)
+
+
+@pytest.mark.parametrize("n,brick,chunk", [(40, (16, 16, 16), 0), (24, (8, 16, 32), 100),
+                                           (33, (32, 8, 8), 7)])
+def test_plan_backprojection_edges(oracle, n, brick, chunk):
+    """Cube edges that are not brick multiples, anisotropic bricks, tiny chunks,
+    with pixel and voxel masks."""
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    print("plan bpj edges", n, brick, chunk, flush=True)
+    ch, x, data = C.small_case(n=n, npix=32, nv=6, dtype=np.float64, radius=9.0)
+    rng = np.random.default_rng(22)
+    y = rng.normal(size=data.shape)
+    mask = rng.uniform(size=data.shape) < 0.2
+    vm = rng.uniform(size=(n, n, n)) < 0.1
+    ref = np.zeros((n, n, n))
+    oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref,
+                  T.InfoArray(y, header=data.header), sun=True, nthread=1, mask=mask)
+    ref[vm] = 0.0
+    P = siddon_lo(data.header, ch, dtype=np.float64, obstacle="sun", mask=mask, voxmask=vm,
+                  bpj_mode="plan", brick=brick, chunk_records=chunk)
+    out = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
+    T.check_status()
+    assert rel_l2(out, ref) <= 1e-12
+
+
+def test_plan_backprojection_time_bins(oracle):
+    from tomograpy_amd.lo_wrapper import Siddon4d
+    ch, x, data = C.small_case(n=20, npix=24, nv=8, dtype=np.float64, radius=9.0)
+    tindex = np.array([0, 2, 1, 2, 0, 1, 1, 2], np.int32)
+    rng = np.random.default_rng(23)
+    y = rng.normal(size=data.shape)
+    ref = np.zeros((20, 20, 20, 3))
+    oracle.siddon(oracle.BPJT, view_records(data.header), map_record(ch), ref,
+                  T.InfoArray(y, header=data.header), nthread=1, tindex=tindex)
+    P = Siddon4d(data.header, ch, tindex=tindex, dtype=np.float64, bpj_mode="plan", brick=(8, 8, 8))
+    out = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
+    assert rel_l2(out, ref) <= 1e-12

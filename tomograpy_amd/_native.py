@@ -55,6 +55,12 @@ class TgCgSeg(ctypes.Structure):
  TG_CG_OP_ZERO_Q) = range(6)
 
 
+class TgBpjPlan(ctypes.Structure):
+    _fields_ = [("brick", _i32 * 3), ("chunk_records", _i32), ("nb_axis", _i32 * 3),
+                ("nbricks", _i32), ("records", _i64), ("chunks", _i64), ("meta", _vp),
+                ("meta_bytes", _sz), ("rec", _vp), ("rec_bytes", _sz)]
+
+
 class TgBatchPlan(ctypes.Structure):
     _fields_ = [("tile_j", _i32), ("blocks", _i32), ("schedule", _i32), ("pad", _i32)]
 
@@ -78,6 +84,11 @@ SIGNATURES = [
     ("tg_cg_dot", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _i32, _vp]),
     ("tg_cg_update", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("tg_cg_direction", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp]),
+    ("tg_bpj_plan_init", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan)]),
+    ("tg_bpj_plan_count", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan), _vp]),
+    ("tg_bpj_plan_build", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan), _vp]),
+    ("tg_bpj_plan_backproject", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan),
+                                               _vp]),
     ("tg_cg_seg_blocks", _i64, [_i64]),
     ("tg_cg_batch", ctypes.c_int, [_i32, _i32, _vp, _i32, _i64, _vp, _vp]),
     ("tg_batch_queue_bytes", _sz, [_i32]),

@@ -1,0 +1,477 @@
+// Brick-privatised backprojection.
+//
+// A scatter of d * w into the cube with one global atomic per Siddon step is
+// bound by atomic throughput.  Here the cube is cut into bricks that fit in
+// LDS.  A one-time, value-independent plan traces every ray once and records,
+// at each brick the ray enters, the traversal state there (voxel index, the
+// three distances D and the parameter ac), grouped by brick.  A
+// backprojection then runs one workgroup per chunk of a brick's records: each
+// lane restores a record, steps through the brick with exactly the
+// operations of the continuous traversal (so every length d is the one the
+// reference computes), adds d * w into the LDS brick, and the workgroup
+// flushes the brick with one coalesced atomic per touched voxel.
+//
+// Plan passes: (1) count records per brick (LDS histogram per workgroup,
+// flushed with one global atomic per touched brick); counts are copied to the
+// host, which computes brick offsets and chunk boundaries; (2) fill: each
+// workgroup counts again, reserves a contiguous range per brick with one
+// global atomic, and writes its records (SoA) into it.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <vector>
+
+#include "siddon_common.h"
+#include "siddon_host.h"
+
+namespace tg {
+
+template <typename T>
+int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a);
+
+namespace {
+
+constexpr int kPlanBlock = 256;
+constexpr int kHistMax = 8192;  // bricks x bins histogrammed in LDS; above: global atomics
+constexpr int kBpjBlock = 1024;
+
+struct BrickGeom {
+  int32_t b[3];       // brick edge per axis
+  int32_t lg[3];      // log2 of the edges
+  int32_t nb[3];      // bricks per axis
+  int32_t nbricks;    // nb0 * nb1 * nb2 * nt
+  int32_t nt;
+  int32_t chunk;      // records per chunk
+};
+
+template <typename T>
+struct RecPtrs {
+  uint32_t *rid;
+  uint32_t *ivp;
+  T *d0, *d1, *d2, *ac;
+};
+
+__device__ __forceinline__ int brick_of(const BrickGeom &g, const int iv[3], int bin) {
+  return ((((iv[0] >> g.lg[0]) * g.nb[1]) + (iv[1] >> g.lg[1])) * g.nb[2] + (iv[2] >> g.lg[2])) +
+         bin * g.nb[0] * g.nb[1] * g.nb[2];
+}
+
+// Traces the ray of (tile, lane) and calls emit(brick, ray) at the first step
+// in every brick.  Same traversal as the projector kernels.
+template <typename T, bool SUN, class F>
+__device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom &g, uint32_t tile,
+                                             F &&emit) {
+  int i, j, t;
+  if (!lane_pixel(a, tile, i, j, t)) return;
+  if (a.pixmask && a.pixmask[i * a.ms[0] + j * a.ms[1] + (int64_t)t * a.ms[2]] != 0) return;
+  const ViewHdr<T> &h = a.views[t];
+  T u0[3];
+  pixel_unit_vector(a, h, t, i, j, u0);
+  Ray<T> r;
+  if (!ray_setup(a, h.M, u0, r)) return;
+  if (!ray_state_ok(r)) {
+    flag_status(a.status, TG_EGEOMETRY);
+    return;
+  }
+  int bin = 0;
+  if (a.kind == TG_BPJT) {
+    bin = a.tindex ? a.tindex[t] : t;
+    if (bin < 0 || bin >= a.nt) {
+      flag_status(a.status, TG_EINVAL);
+      return;
+    }
+  }
+  const uint32_t rid = ((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j;
+  int last = -1;
+  int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
+  while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
+    if (--guard < 0) {
+      flag_status(a.status, TG_EGEOMETRY);
+      break;
+    }
+    const int b = brick_of(g, r.iv, bin);
+    if (b != last) {
+      emit(b, r, rid);
+      last = b;
+    }
+    bool sx, sy, sz;
+    step(r, sx, sy, sz);
+  }
+}
+
+template <typename T, bool SUN>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom g,
+                                                           uint32_t *counts) {
+  __shared__ uint32_t hist[kHistMax];
+  const bool lds = g.nbricks <= kHistMax;
+  if (lds) {
+    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+  }
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &, uint32_t) {
+    if (lds) atomicAdd(&hist[b], 1u);
+    else atomicAdd(&counts[b], 1u);
+  });
+  if (lds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x)
+      if (hist[k]) atomicAdd(&counts[k], hist[k]);
+  }
+}
+
+template <typename T, bool SUN>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom g,
+                                                          const unsigned long long *offsets,
+                                                          uint32_t *cursor, RecPtrs<T> rec) {
+  __shared__ uint32_t slot[kHistMax];
+  const bool lds = g.nbricks <= kHistMax;
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  if (lds) {
+    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) slot[k] = 0;
+    __syncthreads();
+    trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &, uint32_t) { atomicAdd(&slot[b], 1u); });
+    __syncthreads();
+    // reserve one contiguous range per brick for this workgroup's records
+    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x)
+      if (slot[k]) slot[k] = atomicAdd(&cursor[k], slot[k]);
+    __syncthreads();
+  }
+  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &r, uint32_t rid) {
+    const uint32_t s = lds ? atomicAdd(&slot[b], 1u) : atomicAdd(&cursor[b], 1u);
+    const unsigned long long p = offsets[b] + s;
+    rec.rid[p] = rid;
+    rec.ivp[p] = (uint32_t)r.iv[0] | ((uint32_t)r.iv[1] << 10) | ((uint32_t)r.iv[2] << 20);
+    rec.d0[p] = r.D[0];
+    rec.d1[p] = r.D[1];
+    rec.d2[p] = r.D[2];
+    rec.ac[p] = r.ac;
+  });
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_add(T *p, T v) {
+  atomicAdd(p, v);
+}
+
+// One workgroup per chunk of one brick's records.
+template <typename T, bool SUN>
+__global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
+                                                        const unsigned long long *offsets,
+                                                        const unsigned long long *chunk_prefix,
+                                                        RecPtrs<T> rec) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *acc = reinterpret_cast<T *>(smem_raw);
+  const int vol = g.b[0] * g.b[1] * g.b[2];
+  // chunk -> brick (binary search over the chunk prefix, uniform per workgroup)
+  const unsigned long long c = blockIdx.x;
+  int lo = 0, hi = g.nbricks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (chunk_prefix[mid] <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  const int b = lo;
+  const unsigned long long r0 = offsets[b] + (c - chunk_prefix[b]) * (unsigned long long)g.chunk;
+  unsigned long long r1 = r0 + g.chunk;
+  if (r1 > offsets[b + 1]) r1 = offsets[b + 1];
+  const int nsp = g.nb[0] * g.nb[1] * g.nb[2];
+  const int bin = b / nsp;
+  const int bs = b % nsp;
+  const int bz = bs % g.nb[2], by = (bs / g.nb[2]) % g.nb[1], bx = bs / (g.nb[2] * g.nb[1]);
+  const int lo0 = bx << g.lg[0], lo1 = by << g.lg[1], lo2 = bz << g.lg[2];
+  const int e0 = min(g.b[0], (int)a.n[0] - lo0), e1 = min(g.b[1], (int)a.n[1] - lo1),
+            e2 = min(g.b[2], (int)a.n[2] - lo2);
+  for (int k = threadIdx.x; k < vol; k += blockDim.x) acc[k] = (T)0;
+  __syncthreads();
+  const uint32_t per_view = (uint32_t)a.n1 * (uint32_t)a.n2;
+  for (unsigned long long p = r0 + threadIdx.x; p < r1; p += blockDim.x) {
+    const uint32_t rid = rec.rid[p];
+    const int t = (int)(rid / per_view);
+    const uint32_t rem = rid % per_view;
+    const int i = (int)(rem / (uint32_t)a.n2), j = (int)(rem % (uint32_t)a.n2);
+    const int64_t did = i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+    T w = a.data[did];
+    if (a.pixw) w = a.pixw[did] * w;
+    if (w == (T)0) continue;
+    const ViewHdr<T> &h = a.views[t];
+    T u0[3];
+    pixel_unit_vector(a, h, t, i, j, u0);
+    Ray<T> r;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {  // the per-ray constants of C:750-774, as ray_setup computes them
+      const T pk = (u0[k] != 0) ? a.mp[k] / u0[k] : (T)kInf;
+      r.pabs[k] = fabs(pk);
+      r.up[k] = sgn(u0[k]);
+    }
+    const uint32_t v = rec.ivp[p];
+    r.iv[0] = (int)(v & 1023u);
+    r.iv[1] = (int)((v >> 10) & 1023u);
+    r.iv[2] = (int)((v >> 20) & 1023u);
+    r.D[0] = rec.d0[p];
+    r.D[1] = rec.d1[p];
+    r.D[2] = rec.d2[p];
+    r.ac = rec.ac[p];
+    int guard = g.b[0] + g.b[1] + g.b[2] + 2;
+    while (((unsigned)(r.iv[0] - lo0) < (unsigned)e0) & ((unsigned)(r.iv[1] - lo1) < (unsigned)e1) &
+           ((unsigned)(r.iv[2] - lo2) < (unsigned)e2) && (!SUN || outside_sun(h.M, u0, r.ac))) {
+      if (--guard < 0) break;
+      const int l = (((r.iv[0] - lo0) << g.lg[1]) + (r.iv[1] - lo1)) << g.lg[2] | (r.iv[2] - lo2);
+      bool sx, sy, sz;
+      const T d = step(r, sx, sy, sz);
+      lds_add(acc + l, d * w);
+    }
+  }
+  __syncthreads();
+  T *cube = a.cube + (int64_t)bin * a.cs[3];
+  for (int k = threadIdx.x; k < vol; k += blockDim.x) {
+    const T v = acc[k];
+    if (v == (T)0) continue;
+    const int l2 = k & (g.b[2] - 1), l1 = (k >> g.lg[2]) & (g.b[1] - 1), l0 = k >> (g.lg[2] + g.lg[1]);
+    const int64_t i0 = lo0 + l0, i1 = lo1 + l1, i2 = lo2 + l2;
+    if (a.voxmask && a.voxmask[i0 * a.vs[0] + i1 * a.vs[1] + i2 * a.vs[2]]) continue;
+    unsafeAtomicAdd(cube + i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2], v);
+  }
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
+}
+
+int ilog2(int x) {
+  int l = 0;
+  while ((1 << l) < x) l++;
+  return l;
+}
+
+int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
+  memset(&g, 0, sizeof(g));
+  const bool time = (c->kind == TG_PJT || c->kind == TG_BPJT);
+  g.nt = time ? (c->map.nt > 0 ? c->map.nt : 1) : 1;
+  for (int k = 0; k < 3; k++) {
+    int b = p->brick[k];
+    if (b <= 0 || (b & (b - 1))) return set_error(TG_EINVAL, "brick edges must be powers of two");
+    g.b[k] = b;
+    g.lg[k] = ilog2(b);
+    g.nb[k] = (c->map.n[k] + b - 1) / b;
+    if (c->map.n[k] > 1024) return set_error(TG_EINVAL, "plan backprojection supports n <= 1024");
+  }
+  const long long nb = (long long)g.nb[0] * g.nb[1] * g.nb[2] * g.nt;
+  if (nb > (1 << 30)) return set_error(TG_EINVAL, "too many bricks");
+  g.nbricks = (int)nb;
+  g.chunk = p->chunk_records > 0 ? p->chunk_records : 16384;
+  return TG_OK;
+}
+
+struct MetaLayout {
+  size_t counts, offsets, cursor, chunk_prefix, total;
+};
+
+MetaLayout meta_layout(int nbricks) {
+  MetaLayout m;
+  m.counts = 0;
+  m.offsets = align_up(m.counts + 4 * (size_t)nbricks, 256);
+  m.cursor = align_up(m.offsets + 8 * ((size_t)nbricks + 1), 256);
+  m.chunk_prefix = align_up(m.cursor + 4 * (size_t)nbricks, 256);
+  m.total = align_up(m.chunk_prefix + 8 * ((size_t)nbricks + 1), 256);
+  return m;
+}
+
+template <typename T>
+RecPtrs<T> rec_ptrs(const tg_bpj_plan *p) {
+  RecPtrs<T> r;
+  const size_t R = (size_t)p->records;
+  char *base = (char *)p->rec;
+  size_t o = 0;
+  r.rid = (uint32_t *)(base + o);
+  o = align_up(o + 4 * R, 256);
+  r.ivp = (uint32_t *)(base + o);
+  o = align_up(o + 4 * R, 256);
+  r.d0 = (T *)(base + o);
+  o = align_up(o + sizeof(T) * R, 256);
+  r.d1 = (T *)(base + o);
+  o = align_up(o + sizeof(T) * R, 256);
+  r.d2 = (T *)(base + o);
+  o = align_up(o + sizeof(T) * R, 256);
+  r.ac = (T *)(base + o);
+  return r;
+}
+
+template <typename T>
+size_t rec_bytes(size_t R) {
+  return 2 * align_up(4 * R, 256) + 4 * align_up(sizeof(T) * R, 256) + 256;
+}
+
+template <typename T>
+int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
+  KArgs<T> a;
+  int rc = make_args<T>(c, 0, c->nv, a);
+  if (rc) return rc;
+  BrickGeom g;
+  if ((rc = make_geom(c, p, g))) return rc;
+  const MetaLayout m = meta_layout(g.nbricks);
+  if (!p->meta || p->meta_bytes < m.total)
+    return set_error(TG_EINVAL, "plan meta buffer needs %zu bytes", m.total);
+  if ((uint64_t)c->nv * c->n1 * c->n2 > 0xffffffffull)
+    return set_error(TG_EINVAL, "plan backprojection supports < 2^32 rays");
+  char *meta = (char *)p->meta;
+  uint32_t *counts = (uint32_t *)(meta + m.counts);
+  hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, s, counts, g.nbricks);
+  const uint32_t blocks = (uint32_t)c->nv * a.tiles_i * a.tiles_j;
+  if (blocks) {
+    if (c->flags & TG_OBSTACLE_SUN)
+      hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, a, g, counts);
+    else
+      hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, a, g, counts);
+  }
+  if ((rc = hip_check(hipGetLastError(), "plan count launch"))) return rc;
+  std::vector<uint32_t> hc(g.nbricks);
+  if ((rc = hip_check(hipMemcpyAsync(hc.data(), counts, 4 * (size_t)g.nbricks,
+                                     hipMemcpyDeviceToHost, s), "plan counts copy")))
+    return rc;
+  if ((rc = hip_check(hipStreamSynchronize(s), "plan count sync"))) return rc;
+  std::vector<unsigned long long> off(g.nbricks + 1), cp(g.nbricks + 1);
+  unsigned long long tot = 0, ch = 0;
+  for (int b = 0; b < g.nbricks; b++) {
+    off[b] = tot;
+    cp[b] = ch;
+    tot += hc[b];
+    ch += (hc[b] + g.chunk - 1) / g.chunk;
+  }
+  off[g.nbricks] = tot;
+  cp[g.nbricks] = ch;
+  if ((rc = hip_check(hipMemcpyAsync(meta + m.offsets, off.data(), 8 * off.size(),
+                                     hipMemcpyHostToDevice, s), "plan offsets copy")))
+    return rc;
+  if ((rc = hip_check(hipMemcpyAsync(meta + m.chunk_prefix, cp.data(), 8 * cp.size(),
+                                     hipMemcpyHostToDevice, s), "plan chunk copy")))
+    return rc;
+  if ((rc = hip_check(hipStreamSynchronize(s), "plan meta sync"))) return rc;
+  p->records = (int64_t)tot;
+  p->chunks = (int64_t)ch;
+  p->rec_bytes = rec_bytes<T>(tot);
+  for (int k = 0; k < 3; k++) p->nb_axis[k] = g.nb[k];
+  p->nbricks = g.nbricks;
+  return TG_OK;
+}
+
+template <typename T>
+int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
+  KArgs<T> a;
+  int rc = make_args<T>(c, 0, c->nv, a);
+  if (rc) return rc;
+  BrickGeom g;
+  if ((rc = make_geom(c, p, g))) return rc;
+  const MetaLayout m = meta_layout(g.nbricks);
+  if (!p->meta || p->meta_bytes < m.total) return set_error(TG_EINVAL, "plan meta buffer too small");
+  if (!p->rec || p->rec_bytes < rec_bytes<T>((size_t)p->records))
+    return set_error(TG_EINVAL, "plan record buffer needs %zu bytes", rec_bytes<T>((size_t)p->records));
+  char *meta = (char *)p->meta;
+  uint32_t *cursor = (uint32_t *)(meta + m.cursor);
+  const unsigned long long *offsets = (const unsigned long long *)(meta + m.offsets);
+  hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, s, cursor, g.nbricks);
+  const uint32_t blocks = (uint32_t)c->nv * a.tiles_i * a.tiles_j;
+  RecPtrs<T> rp = rec_ptrs<T>(p);
+  if (blocks) {
+    if (c->flags & TG_OBSTACLE_SUN)
+      hipLaunchKernelGGL((k_plan_fill<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, a, g,
+                         offsets, cursor, rp);
+    else
+      hipLaunchKernelGGL((k_plan_fill<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, a, g,
+                         offsets, cursor, rp);
+  }
+  return hip_check(hipGetLastError(), "plan fill launch");
+}
+
+template <typename T, bool SUN>
+int launch_plan_bpj(const KArgs<T> &a, const BrickGeom &g, const tg_bpj_plan *p, hipStream_t s) {
+  const MetaLayout m = meta_layout(g.nbricks);
+  const char *meta = (const char *)p->meta;
+  const size_t lds = sizeof(T) * (size_t)g.b[0] * g.b[1] * g.b[2];
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_plan_bpj<T, SUN>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return hip_check(e, "plan bpj LDS attribute");
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_plan_bpj<T, SUN>), dim3((uint32_t)p->chunks), dim3(kBpjBlock), lds, s, a, g,
+                     (const unsigned long long *)(meta + m.offsets),
+                     (const unsigned long long *)(meta + m.chunk_prefix), rec_ptrs<T>(p));
+  return hip_check(hipGetLastError(), "plan bpj launch");
+}
+
+template <typename T>
+int plan_bpj(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
+  if (c->kind != TG_BPJ && c->kind != TG_BPJT)
+    return set_error(TG_EINVAL, "plan backprojection needs kind bpj or bpjt");
+  KArgs<T> a;
+  int rc = make_args<T>(c, 0, c->nv, a);
+  if (rc) return rc;
+  if (!c->cube || !c->data) return set_error(TG_EINVAL, "cube and data pointers are required");
+  BrickGeom g;
+  if ((rc = make_geom(c, p, g))) return rc;
+  if (g.nbricks != p->nbricks) return set_error(TG_EINVAL, "plan was built for another geometry");
+  if (!p->meta || p->meta_bytes < meta_layout(g.nbricks).total || !p->rec ||
+      p->rec_bytes < rec_bytes<T>((size_t)p->records))
+    return set_error(TG_EINVAL, "plan buffers missing or too small");
+  const size_t lds = sizeof(T) * (size_t)g.b[0] * g.b[1] * g.b[2];
+  if (lds > 160 * 1024) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
+  if (p->chunks == 0) return TG_OK;
+  if (c->flags & TG_OBSTACLE_SUN) return launch_plan_bpj<T, true>(a, g, p, s);
+  return launch_plan_bpj<T, false>(a, g, p, s);
+}
+
+}  // namespace
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" {
+
+int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p) {
+  if (!c || !p) return set_error(TG_EINVAL, "null argument");
+  if (p->brick[0] == 0 && p->brick[1] == 0 && p->brick[2] == 0) {
+    // default: 128 KiB of LDS per brick
+    p->brick[0] = c->dtype == TG_F64 ? 16 : 32;
+    p->brick[1] = 32;
+    p->brick[2] = 32;
+  }
+  BrickGeom g;
+  int rc = make_geom(c, p, g);
+  if (rc) return rc;
+  p->chunk_records = g.chunk;
+  p->nbricks = g.nbricks;
+  for (int k = 0; k < 3; k++) p->nb_axis[k] = g.nb[k];
+  p->meta_bytes = meta_layout(g.nbricks).total;
+  p->records = 0;
+  p->chunks = 0;
+  p->rec_bytes = 0;
+  return TG_OK;
+}
+
+int tg_bpj_plan_count(const tg_call *c, tg_bpj_plan *p, void *stream) {
+  if (!c || !p) return set_error(TG_EINVAL, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->dtype == TG_F32) return plan_count<float>(c, p, s);
+  if (c->dtype == TG_F64) return plan_count<double>(c, p, s);
+  return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+int tg_bpj_plan_build(const tg_call *c, const tg_bpj_plan *p, void *stream) {
+  if (!c || !p) return set_error(TG_EINVAL, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->dtype == TG_F32) return plan_build<float>(c, p, s);
+  if (c->dtype == TG_F64) return plan_build<double>(c, p, s);
+  return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+int tg_bpj_plan_backproject(const tg_call *c, const tg_bpj_plan *p, void *stream) {
+  if (!c || !p) return set_error(TG_EINVAL, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->dtype == TG_F32) return plan_bpj<float>(c, p, s);
+  if (c->dtype == TG_F64) return plan_bpj<double>(c, p, s);
+  return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+}  // extern "C"

@@ -71,7 +71,7 @@ class Siddon:
 
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
                  voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
-                 tile_j=0, device=None):
+                 tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0):
         self.data_header = list(data_header)
         self.map_header = dict(cube_header)
         check_headers(self.data_header, self.map_header)
@@ -113,8 +113,49 @@ class Siddon:
         self.xout = stack_buffer(self.n1, self.n2, self.nv, self.tdtype, self.dev)
         self.status = S._status_tensor(self.dev)
         self._ws = None
-        self._fwd_call = None
-        self._bwd_call = None
+        if bpj_mode not in ("auto", "plan", "atomic"):
+            raise ValueError("bpj_mode must be 'auto', 'plan' or 'atomic'")
+        self.bpj_mode = bpj_mode
+        self.brick = brick
+        self.chunk_records = int(chunk_records)
+        self._plan = None
+        self._plan_bufs = None
+
+    # -- brick-privatised backprojection plan ------------------------------
+    def use_plan(self):
+        """Whether backprojections go through the brick plan."""
+        if self.deterministic or self.bpj_mode == "atomic":
+            return False
+        if self.bpj_mode == "plan":
+            return True
+        return int(np.prod(self.cube_shape)) >= 64 ** 3 and max(self.cube_shape) <= 1024
+
+    def build_plan(self):
+        """Builds (once) the value-independent brick plan of this geometry."""
+        if self._plan is not None:
+            return self._plan
+        lib = N.lib()
+        c = self._call(N.TG_BPJ, self.xin, self.xout)
+        p = N.TgBpjPlan()
+        if self.brick is not None:
+            for k in range(3):
+                p.brick[k] = int(self.brick[k])
+        p.chunk_records = self.chunk_records
+        N.check(lib.tg_bpj_plan_init(ctypes.byref(c), ctypes.byref(p)))
+        meta = torch.empty(max(int(p.meta_bytes), 1), dtype=torch.uint8, device=self.dev)
+        p.meta = meta.data_ptr()
+        p.meta_bytes = meta.numel()
+        N.check(lib.tg_bpj_plan_count(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
+        rec = torch.empty(max(int(p.rec_bytes), 1), dtype=torch.uint8, device=self.dev)
+        p.rec = rec.data_ptr()
+        p.rec_bytes = rec.numel()
+        N.check(lib.tg_bpj_plan_build(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
+        self._plan = p
+        self._plan_bufs = (meta, rec)
+        return p
+
+    def plan_bytes(self):
+        return 0 if self._plan_bufs is None else sum(b.numel() for b in self._plan_bufs)
 
     # -- calls -------------------------------------------------------------
     def _call(self, kind, cube, data):
@@ -172,17 +213,24 @@ class Siddon:
         y = self._as_stack(y)
         if self.vox_weight is not None:
             tmp = torch.zeros_like(out)
-            c = self._call(N.TG_BPJ, tmp, y)
-            N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
+            self._backproject_raw(y, tmp)
             sc = S.make_call(N.TG_BPJ, self.code, y.shape, y.stride(), tmp.shape, tmp.stride(),
                              self.map_rec, self.views, tmp.data_ptr(), y.data_ptr(),
                              vox_weight=self.vox_weight)
             N.check(N.lib().tg_scale_cube(ctypes.byref(sc), S.stream_handle()))
             out.add_(tmp)
             return out
-        c = self._call(N.TG_BPJ, out, y)
-        N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
+        self._backproject_raw(y, out)
         return out
+
+    def _backproject_raw(self, y, out):
+        c = self._call(N.TG_BPJ, out, y)
+        if self.use_plan():
+            p = self.build_plan()
+            N.check(N.lib().tg_bpj_plan_backproject(ctypes.byref(c), ctypes.byref(p),
+                                                    S.stream_handle()))
+        else:
+            N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
 
     def matvec(self, x):
         """P x into the cached image buffer (zeroed first), lo_wrapper.py:23-28."""
@@ -262,8 +310,7 @@ class Siddon4d(Siddon):
         y = self._as_stack(y)
         if self.vox_weight is not None:
             raise NotImplementedError("voxel weights with time bins")
-        c = self._call(N.TG_BPJ, out, y)
-        N.check(N.lib().tg_conic_full_projector(ctypes.byref(c), S.stream_handle()))
+        self._backproject_raw(y, out)
         return out
 
     def count_steps(self):

@@ -155,6 +155,9 @@ typedef struct tg_bpj_plan {
   int32_t chunk_records; /* records per workgroup chunk; 0 = default */
   int32_t nb_axis[3];    /* bricks per axis (set by tg_bpj_plan_init) */
   int32_t nbricks;       /* bricks x time bins (set by tg_bpj_plan_init) */
+  int32_t order;         /* 0 = records of a brick in trace order, 1 = permuted (spreads the
+                            rays of a wavefront over the brick) */
+  int32_t pad;
   int64_t records;       /* set by tg_bpj_plan_count */
   int64_t chunks;        /* set by tg_bpj_plan_count */
   void *meta;            /* device, caller-owned, meta_bytes */

@@ -163,3 +163,32 @@ def test_batch_job_table_layout(dtype_code, schedule):
         assert tb == begin, (q, k)
         assert cube == calls[k].cube, (q, k)
         begin += expect_tiles[k]
+
+
+@pytest.mark.parametrize("dtype_code,n,brick", [(N.TG_F32, 40, (0, 0, 0)), (N.TG_F64, 70, (0, 0, 0)),
+                                                (N.TG_F32, 33, (8, 16, 32))])
+def test_bpj_plan_init_fields(dtype_code, n, brick):
+    """tg_bpj_plan_init (host only) fills the fields where the ctypes mirror expects them."""
+    call = N.TgCall()
+    call.dtype = dtype_code
+    call.kind = N.TG_BPJT
+    call.map.n[0] = call.map.n[1] = call.map.n[2] = n
+    call.map.nt = 3
+    call.n1 = call.n2 = 16
+    call.nv = 4
+    p = N.TgBpjPlan()
+    for k in range(3):
+        p.brick[k] = brick[k]
+    p.order = 1
+    N.check(N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)))
+    b = list(p.brick)
+    if brick == (0, 0, 0):
+        assert b == [32, 32, 16]  # 128 KiB of double accumulators
+    nb = [(n + b[k] - 1) // b[k] for k in range(3)]
+    assert list(p.nb_axis) == nb
+    assert p.nbricks == nb[0] * nb[1] * nb[2] * 3
+    assert p.order == 1 and p.records == 0 and p.chunks == 0
+    assert p.chunk_records == 16384
+    assert p.meta_bytes >= p.nbricks * (4 + 8 + 4 + 8 + 4)
+    p.brick[0] = 12
+    assert N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)) == N.TG_EINVAL

@@ -71,7 +71,8 @@ class Siddon:
 
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
                  voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
-                 tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0):
+                 tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0,
+                 plan_order=1):
         self.data_header = list(data_header)
         self.map_header = dict(cube_header)
         check_headers(self.data_header, self.map_header)
@@ -118,6 +119,7 @@ class Siddon:
         self.bpj_mode = bpj_mode
         self.brick = brick
         self.chunk_records = int(chunk_records)
+        self.plan_order = int(plan_order)
         self._plan = None
         self._plan_bufs = None
 
@@ -141,6 +143,7 @@ class Siddon:
             for k in range(3):
                 p.brick[k] = int(self.brick[k])
         p.chunk_records = self.chunk_records
+        p.order = self.plan_order
         N.check(lib.tg_bpj_plan_init(ctypes.byref(c), ctypes.byref(p)))
         meta = torch.empty(max(int(p.meta_bytes), 1), dtype=torch.uint8, device=self.dev)
         p.meta = meta.data_ptr()

@@ -31,14 +31,14 @@ namespace tg {
 // Ray-driven kernels
 // ---------------------------------------------------------------------------
 
-template <typename T, bool SUN, bool VOX>
+template <typename T, bool SUN, bool VOX, bool FAST>
 __global__ __launch_bounds__(256) void k_project(KArgs<T> a) {
-  project_tile<T, SUN, VOX>(a, (a.ray_i >= 0) ? 0u : xcd_remap(blockIdx.x, gridDim.x));
+  project_tile<T, SUN, VOX, FAST>(a, (a.ray_i >= 0) ? 0u : xcd_remap(blockIdx.x, gridDim.x));
 }
 
-template <typename T, bool SUN, bool VOX>
+template <typename T, bool SUN, bool VOX, bool FAST>
 __global__ __launch_bounds__(256) void k_backproject(KArgs<T> a) {
-  backproject_tile<T, SUN, VOX>(a, (a.ray_i >= 0) ? 0u : xcd_remap(blockIdx.x, gridDim.x));
+  backproject_tile<T, SUN, VOX, FAST>(a, (a.ray_i >= 0) ? 0u : xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Step count per ray (value independent).
@@ -219,12 +219,19 @@ uint32_t grid_blocks(const KArgs<T> &a) {
   return (uint32_t)(a.t1 - a.t0) * (uint32_t)a.tiles_i * (uint32_t)a.tiles_j;
 }
 
+template <typename T, bool SUN, bool VOX, bool FAST>
+void launch_siddon_f(const KArgs<T> &a, uint32_t blocks, hipStream_t s, bool bwd) {
+  if (bwd)
+    hipLaunchKernelGGL((k_backproject<T, SUN, VOX, FAST>), dim3(blocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_project<T, SUN, VOX, FAST>), dim3(blocks), dim3(256), 0, s, a);
+}
+
 template <typename T, bool SUN, bool VOX>
 void launch_siddon(const KArgs<T> &a, uint32_t blocks, hipStream_t s, bool bwd) {
-  if (bwd)
-    hipLaunchKernelGGL((k_backproject<T, SUN, VOX>), dim3(blocks), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_project<T, SUN, VOX>), dim3(blocks), dim3(256), 0, s, a);
+  const bool fast = walk_fast_ok(a.n, a.cs) && (!VOX || walk_fast_ok(a.n, a.vs));
+  if (fast) launch_siddon_f<T, SUN, VOX, true>(a, blocks, s, bwd);
+  else launch_siddon_f<T, SUN, VOX, false>(a, blocks, s, bwd);
 }
 
 template <typename T>

@@ -35,6 +35,8 @@ struct JobDesc {
   unsigned long long tile_begin;
   uint32_t sun;
   uint32_t vox;
+  uint32_t fast;
+  uint32_t pad;
 };
 
 size_t job_bytes() {
@@ -43,26 +45,33 @@ size_t job_bytes() {
                   16);
 }
 
-template <typename T, bool BWD>
-__device__ __forceinline__ void run_tile(const KArgs<T> &a, uint32_t sun, uint32_t vox,
-                                         uint32_t tile) {
+template <typename T, bool BWD, bool FAST>
+__device__ __forceinline__ void run_tile_f(const KArgs<T> &a, uint32_t sun, uint32_t vox,
+                                           uint32_t tile) {
   if (BWD) {
     if (sun) {
-      if (vox) backproject_tile<T, true, true>(a, tile);
-      else backproject_tile<T, true, false>(a, tile);
+      if (vox) backproject_tile<T, true, true, FAST>(a, tile);
+      else backproject_tile<T, true, false, FAST>(a, tile);
     } else {
-      if (vox) backproject_tile<T, false, true>(a, tile);
-      else backproject_tile<T, false, false>(a, tile);
+      if (vox) backproject_tile<T, false, true, FAST>(a, tile);
+      else backproject_tile<T, false, false, FAST>(a, tile);
     }
   } else {
     if (sun) {
-      if (vox) project_tile<T, true, true>(a, tile);
-      else project_tile<T, true, false>(a, tile);
+      if (vox) project_tile<T, true, true, FAST>(a, tile);
+      else project_tile<T, true, false, FAST>(a, tile);
     } else {
-      if (vox) project_tile<T, false, true>(a, tile);
-      else project_tile<T, false, false>(a, tile);
+      if (vox) project_tile<T, false, true, FAST>(a, tile);
+      else project_tile<T, false, false, FAST>(a, tile);
     }
   }
+}
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void run_tile(const KArgs<T> &a, uint32_t sun, uint32_t vox,
+                                         uint32_t fast, uint32_t tile) {
+  if (fast) run_tile_f<T, BWD, true>(a, sun, vox, tile);
+  else run_tile_f<T, BWD, false>(a, sun, vox, tile);
 }
 
 template <typename T, bool BWD>
@@ -83,7 +92,7 @@ __global__ __launch_bounds__(256) void k_batch(const JobDesc<T> *__restrict__ jo
       else hi = mid - 1;
     }
     const JobDesc<T> &jd = jobs[lo];
-    run_tile<T, BWD>(jd.a, jd.sun, jd.vox, (uint32_t)(tile - jd.tile_begin));
+    run_tile<T, BWD>(jd.a, jd.sun, jd.vox, jd.fast, (uint32_t)(tile - jd.tile_begin));
   }
 }
 
@@ -135,6 +144,8 @@ int run_batch(const tg_call *calls, int njobs, const tg_batch_plan *plan, void *
     jd.tile_begin = begin;
     jd.sun = (calls[k].flags & TG_OBSTACLE_SUN) ? 1u : 0u;
     jd.vox = (calls[k].voxmask != nullptr || (!bwd && calls[k].vox_weight != nullptr)) ? 1u : 0u;
+    jd.fast = (walk_fast_ok(args[k].n, args[k].cs) && (!jd.vox || walk_fast_ok(args[k].n, args[k].vs)))
+                  ? 1u : 0u;
     memcpy(&table[q], &jd, sizeof(jd));
     begin += tiles[k];
   }

@@ -233,6 +233,114 @@ __device__ __forceinline__ T step(Ray<T> &r, bool &sx, bool &sy, bool &sz) {
   return d;
 }
 
+// ---------------------------------------------------------------------------
+// Counter form of the traversal used by the hot loops.
+//
+// For a ray that starts inside the map, r_k counts the voxels left along axis
+// k before the ray leaves [0, n_k): n_k - iv_k when stepping up, iv_k + 1 when
+// stepping down, a large constant when the axis never steps (update 0, the
+// cascade can still pick it; iv_k does not move then).  Each step along k
+// decrements r_k, and the ray is inside exactly when every r_k > 0 -- the
+// in_map test of C:948-953.  The element offset of the current voxel is
+// base + sum_k r_k m_k with m_k = -update_k * stride_k, evaluated with 24-bit
+// multiply-adds in 32-bit modular arithmetic (FAST) or in 64 bits.
+// ---------------------------------------------------------------------------
+constexpr int kNoStep = 1 << 22;
+
+__device__ __forceinline__ float tmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double tmin(double a, double b) { return fmin(a, b); }
+
+template <bool FAST>
+struct Walk;
+
+template <>
+struct Walk<true> {
+  int r[3];
+  int m[3];
+  uint32_t base;
+  __device__ __forceinline__ uint32_t off() const {
+    return base + (uint32_t)__mul24(r[0], m[0]) + (uint32_t)__mul24(r[1], m[1]) +
+           (uint32_t)__mul24(r[2], m[2]);
+  }
+};
+
+template <>
+struct Walk<false> {
+  int r[3];
+  int64_t m[3];
+  int64_t base;
+  __device__ __forceinline__ int64_t off() const {
+    return base + r[0] * m[0] + r[1] * m[1] + r[2] * m[2];
+  }
+};
+
+// Counters of a ray whose voxel iv is inside [0, n) (check in_map first), for
+// an array with element strides st.
+template <bool FAST, typename T, typename S>
+__device__ __forceinline__ void walk_init(const Ray<T> &r, const uint32_t n[3], const S st[3],
+                                          Walk<FAST> &w) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    w.r[k] = r.up[k] > 0 ? (int)n[k] - r.iv[k] : (r.up[k] < 0 ? r.iv[k] + 1 : kNoStep);
+    w.m[k] = -(r.up[k] * st[k]);
+  }
+  if (FAST) {
+    uint32_t lin = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) lin += (uint32_t)r.iv[k] * (uint32_t)st[k];
+    w.base = lin - ((uint32_t)(w.r[0] * w.m[0]) + (uint32_t)(w.r[1] * w.m[1]) +
+                    (uint32_t)(w.r[2] * w.m[2]));
+  } else {
+    int64_t lin = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) lin += (int64_t)r.iv[k] * (int64_t)st[k];
+    w.base = lin - ((int64_t)w.r[0] * w.m[0] + (int64_t)w.r[1] * w.m[1] + (int64_t)w.r[2] * w.m[2]);
+  }
+}
+
+template <bool FAST>
+__device__ __forceinline__ bool walk_inside(const Walk<FAST> &w) {
+  return min(min(w.r[0], w.r[1]), w.r[2]) > 0;
+}
+
+// One step of C:811-912 on (D, ac) with the counters of up to two walks
+// (cube and voxel operands) instead of iv.  Same floating-point operations as
+// step(): the minimum is taken with min3 (exact: no NaN reaches here), the
+// length is D of the first stepping axis in x, y, z order.
+template <typename T, bool AC, bool FAST, bool TWO>
+__device__ __forceinline__ T step_walk(Ray<T> &r, Walk<FAST> &w, Walk<FAST> &w2) {
+  const T dmin = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
+  const bool sx = (r.D[0] == dmin);
+  const bool sy = (r.D[1] == dmin);
+  const bool sz = (r.D[2] == dmin);
+  const T d = sx ? r.D[0] : (sy ? r.D[1] : r.D[2]);
+  if (AC) r.ac += d;
+  r.D[0] = sx ? r.pabs[0] : r.D[0] - d;
+  r.D[1] = sy ? r.pabs[1] : r.D[1] - d;
+  r.D[2] = sz ? r.pabs[2] : r.D[2] - d;
+  w.r[0] -= sx;
+  w.r[1] -= sy;
+  w.r[2] -= sz;
+  if (TWO) {
+    w2.r[0] -= sx;
+    w2.r[1] -= sy;
+    w2.r[2] -= sz;
+  }
+  return d;
+}
+
+// Whether the 32-bit walk applies: every |stride| < 2^23 (24-bit multiplies)
+// and every offset of the 3-D part below 2^31.
+inline bool walk_fast_ok(const uint32_t n[3], const int64_t st[3]) {
+  int64_t ext = 1;
+  for (int k = 0; k < 3; k++) {
+    const int64_t a = st[k] < 0 ? -st[k] : st[k];
+    if (a >= (1 << 23)) return false;
+    ext += (int64_t)(n[k] - 1) * a;
+  }
+  return ext < (1ll << 31);
+}
+
 // Workgroup -> logical tile mapping that gives each XCD a contiguous range of
 // logical tiles (workgroups are dispatched round-robin over the 8 XCDs), so
 // one XCD's L2 serves neighbouring rays of the same views.

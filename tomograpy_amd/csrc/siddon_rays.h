@@ -1,6 +1,7 @@
 // Per-ray bodies of the Siddon projector and backprojector, shared by the
 // one-job kernels (siddon.hip) and the persistent multi-job kernel
 // (siddon_batch.hip).  `tile` is the logical workgroup tile of the job.
+// FAST selects the 32-bit counter walk (walk_fast_ok on the host).
 #pragma once
 
 #include "siddon_common.h"
@@ -9,7 +10,7 @@ namespace tg {
 
 // Forward projection (pj / pjt), optionally with the voxel operands of
 // O.P.M (weight, mask) and the pixel weight O.
-template <typename T, bool SUN, bool VOX>
+template <typename T, bool SUN, bool VOX, bool FAST>
 __device__ __forceinline__ void project_tile(const KArgs<T> &a, uint32_t tile) {
   int i, j, t;
   if (!lane_pixel(a, tile, i, j, t)) return;
@@ -32,34 +33,29 @@ __device__ __forceinline__ void project_tile(const KArgs<T> &a, uint32_t tile) {
   }
   int64_t toff;
   if (!time_offset(a, t, toff)) return;
-  const T *cube = a.cube + toff;
-  int64_t lin = r.iv[0] * a.cs[0] + r.iv[1] * a.cs[1] + r.iv[2] * a.cs[2];
-  const int64_t st0 = r.up[0] * a.cs[0], st1 = r.up[1] * a.cs[1], st2 = r.up[2] * a.cs[2];
-  int64_t lm = 0, sm0 = 0, sm1 = 0, sm2 = 0;
-  if (VOX) {
-    lm = r.iv[0] * a.vs[0] + r.iv[1] * a.vs[1] + r.iv[2] * a.vs[2];
-    sm0 = r.up[0] * a.vs[0];
-    sm1 = r.up[1] * a.vs[1];
-    sm2 = r.up[2] * a.vs[2];
-  }
   T acc = (a.pixw || zero_out) ? (T)0 : a.data[did];
-  int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
-  while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
-    if (--guard < 0) {
-      flag_status(a.status, TG_EGEOMETRY);
-      break;
+  if (in_map(a, r)) {
+    const T *cube = a.cube + toff;
+    Walk<FAST> w, wm;
+    walk_init<FAST>(r, a.n, a.cs, w);
+    if (VOX) walk_init<FAST>(r, a.n, a.vs, wm);
+    int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
+    while (walk_inside(w) && (!SUN || outside_sun(h.M, u0, r.ac))) {
+      if (--guard < 0) {
+        flag_status(a.status, TG_EGEOMETRY);
+        break;
+      }
+      const auto lin = w.off();
+      T v = cube[lin];
+      bool keep = true;
+      if (VOX) {
+        const auto lm = wm.off();
+        if (a.voxw) v = a.voxw[lm] * v;
+        if (a.voxmask) keep = (a.voxmask[lm] == 0);
+      }
+      const T d = step_walk<T, SUN, FAST, VOX>(r, w, wm);
+      if (keep) acc = acc + d * v;
     }
-    bool sx, sy, sz;
-    const T d = step(r, sx, sy, sz);
-    T v = cube[lin];
-    bool keep = true;
-    if (VOX) {
-      if (a.voxw) v = a.voxw[lm] * v;
-      if (a.voxmask) keep = (a.voxmask[lm] == 0);
-      lm += (sx ? sm0 : 0) + (sy ? sm1 : 0) + (sz ? sm2 : 0);
-    }
-    if (keep) acc = acc + d * v;
-    lin += (sx ? st0 : 0) + (sy ? st1 : 0) + (sz ? st2 : 0);
   }
   if (a.pixw)
     a.data[did] = (zero_out ? (T)0 : a.data[did]) + a.pixw[did] * acc;
@@ -73,7 +69,7 @@ __device__ __forceinline__ void atomic_add(T *p, T v) {
 }
 
 // Backprojection (bpj / bpjt) as an atomic scatter.
-template <typename T, bool SUN, bool VOX>
+template <typename T, bool SUN, bool VOX, bool FAST>
 __device__ __forceinline__ void backproject_tile(const KArgs<T> &a, uint32_t tile) {
   int i, j, t;
   if (!lane_pixel(a, tile, i, j, t)) return;
@@ -95,31 +91,24 @@ __device__ __forceinline__ void backproject_tile(const KArgs<T> &a, uint32_t til
   }
   int64_t toff;
   if (!time_offset(a, t, toff)) return;
+  if (!in_map(a, r)) return;
   T *cube = a.cube + toff;
-  int64_t lin = r.iv[0] * a.cs[0] + r.iv[1] * a.cs[1] + r.iv[2] * a.cs[2];
-  const int64_t st0 = r.up[0] * a.cs[0], st1 = r.up[1] * a.cs[1], st2 = r.up[2] * a.cs[2];
-  int64_t lm = 0, sm0 = 0, sm1 = 0, sm2 = 0;
-  if (VOX) {
-    lm = r.iv[0] * a.vs[0] + r.iv[1] * a.vs[1] + r.iv[2] * a.vs[2];
-    sm0 = r.up[0] * a.vs[0];
-    sm1 = r.up[1] * a.vs[1];
-    sm2 = r.up[2] * a.vs[2];
-  }
+  Walk<FAST> wk, wm;
+  walk_init<FAST>(r, a.n, a.cs, wk);
+  if (VOX) walk_init<FAST>(r, a.n, a.vs, wm);
   int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
-  while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
+  while (walk_inside(wk) && (!SUN || outside_sun(h.M, u0, r.ac))) {
     if (--guard < 0) {
       flag_status(a.status, TG_EGEOMETRY);
       break;
     }
-    bool sx, sy, sz;
-    const T d = step(r, sx, sy, sz);
+    const auto lin = wk.off();
     bool keep = true;
     if (VOX) {
-      if (a.voxmask) keep = (a.voxmask[lm] == 0);
-      lm += (sx ? sm0 : 0) + (sy ? sm1 : 0) + (sz ? sm2 : 0);
+      if (a.voxmask) keep = (a.voxmask[wm.off()] == 0);
     }
+    const T d = step_walk<T, SUN, FAST, VOX>(r, wk, wm);
     if (keep) atomic_add(cube + lin, d * w);
-    lin += (sx ? st0 : 0) + (sy ? st1 : 0) + (sz ? st2 : 0);
   }
 }
 

@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cg-iters", type=int, default=0, help="also time this many CG iterations")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -116,6 +118,15 @@ def main():
 
     x = x_true.clone()
     stream = torch.cuda.current_stream()
+    if args.copy_calibration_mib:
+        # a copy kernel that reads and writes exactly this many bytes once
+        n = args.copy_calibration_mib << 18
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def step():
         P.xout.zero_()

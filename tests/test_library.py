@@ -183,12 +183,12 @@ def test_bpj_plan_init_fields(dtype_code, n, brick):
     N.check(N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)))
     b = list(p.brick)
     if brick == (0, 0, 0):
-        assert b == [32, 32, 16]  # 128 KiB of double accumulators
+        assert b == [16, 32, 32]  # 128 KiB of double accumulators
     nb = [(n + b[k] - 1) // b[k] for k in range(3)]
     assert list(p.nb_axis) == nb
     assert p.nbricks == nb[0] * nb[1] * nb[2] * 3
     assert p.order == 1 and p.records == 0 and p.chunks == 0
-    assert p.chunk_records == 16384
+    assert p.chunk_records == 32768
     assert p.meta_bytes >= p.nbricks * (4 + 8 + 4 + 8 + 4)
     p.brick[0] = 12
     assert N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)) == N.TG_EINVAL

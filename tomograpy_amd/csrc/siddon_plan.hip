@@ -239,14 +239,28 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
     r.D[1] = rec.d1[p];
     r.D[2] = rec.d2[p];
     r.ac = rec.ac[p];
+    // counters to the faces of brick & map (the record's voxel is inside both)
+    Walk<true> wl;
+    {
+      const int lo[3] = {lo0, lo1, lo2}, ext[3] = {e0, e1, e2};
+      // byte strides of the double accumulators
+      const int ls[3] = {g.b[1] * g.b[2] * (int)sizeof(Acc), g.b[2] * (int)sizeof(Acc), (int)sizeof(Acc)};
+      uint32_t l0 = 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        wl.r[k] = r.up[k] > 0 ? lo[k] + ext[k] - r.iv[k] : (r.up[k] < 0 ? r.iv[k] - lo[k] + 1 : kNoStep);
+        wl.m[k] = -(r.up[k] * ls[k]);
+        l0 += (uint32_t)((r.iv[k] - lo[k]) * ls[k]);
+      }
+      wl.base = l0 - ((uint32_t)(wl.r[0] * wl.m[0]) + (uint32_t)(wl.r[1] * wl.m[1]) +
+                      (uint32_t)(wl.r[2] * wl.m[2]));
+    }
     int guard = g.b[0] + g.b[1] + g.b[2] + 2;
-    while (((unsigned)(r.iv[0] - lo0) < (unsigned)e0) & ((unsigned)(r.iv[1] - lo1) < (unsigned)e1) &
-           ((unsigned)(r.iv[2] - lo2) < (unsigned)e2) && (!SUN || outside_sun(h.M, u0, r.ac))) {
+    while (walk_inside(wl) && (!SUN || outside_sun(h.M, u0, r.ac))) {
       if (--guard < 0) break;
-      const int l = (((r.iv[0] - lo0) << g.lg[1]) + (r.iv[1] - lo1)) << g.lg[2] | (r.iv[2] - lo2);
-      bool sx, sy, sz;
-      const T d = step(r, sx, sy, sz);
-      atomicAdd(acc + l, (Acc)(d * w));
+      const uint32_t l = wl.off();
+      const T d = step_walk<T, SUN, true, false>(r, wl, wl);
+      atomicAdd((Acc *)((unsigned char *)acc + l), (Acc)(d * w));
     }
   }
   __syncthreads();
@@ -287,7 +301,7 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
   const long long nb = (long long)g.nb[0] * g.nb[1] * g.nb[2] * g.nt;
   if (nb > (1 << 30)) return set_error(TG_EINVAL, "too many bricks");
   g.nbricks = (int)nb;
-  g.chunk = p->chunk_records > 0 ? p->chunk_records : 16384;
+  g.chunk = p->chunk_records > 0 ? p->chunk_records : 32768;
   g.packed = (c->n1 <= 1024 && c->n2 <= 1024 && c->nv <= 4096) ? 1 : 0;
   g.shuffle = p->order == 1 ? 1 : 0;
   return TG_OK;
@@ -483,10 +497,10 @@ extern "C" {
 int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p) {
   if (!c || !p) return set_error(TG_EINVAL, "null argument");
   if (p->brick[0] == 0 && p->brick[1] == 0 && p->brick[2] == 0) {
-    // default: 32 x 32 x 16 voxels = 128 KiB of double accumulators
-    p->brick[0] = 32;
+    // default: 16 x 32 x 32 voxels = 128 KiB of double accumulators
+    p->brick[0] = 16;
     p->brick[1] = 32;
-    p->brick[2] = 16;
+    p->brick[2] = 32;
   }
   BrickGeom g;
   int rc = make_geom(c, p, g);

@@ -48,6 +48,23 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(kernel, args, plan_info):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when this run is the workload that summary measured
+    (counters cannot be read from inside the timed process)."""
+    path = os.path.join(ROOT, "profiles", "siddon_c2_pair_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    w = rec["workload"]
+    if (w["n"], w["views"], w["pix"], w["dtype"]) != (args.n, args.views, args.pix, args.dtype):
+        return None
+    if plan_info is not None and list(w.get("brick", [])) != plan_info["brick"]:
+        return None
+    k = rec["kernels"].get(kernel)
+    return None if k is None else k["hbm_bytes_per_launch"]
+
+
 def cpu_baseline(ch, hs, cube_np, data_np, target_s):
     """Oracle fwd+bwd on a bounded subset of the views, all host threads."""
     from oracle import oracle as O
@@ -119,12 +136,15 @@ def main():
     x = x_true.clone()
     stream = torch.cuda.current_stream()
     if args.copy_calibration_mib:
-        # a copy kernel that reads and writes exactly this many bytes once
+        # copies that read and write exactly this many bytes once: the runtime's
+        # buffer copy (dwordx4 loads) and an elementwise kernel (out = src * 1)
         n = args.copy_calibration_mib << 18
         src = torch.ones(n, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
 
@@ -211,7 +231,8 @@ def main():
                    "backprojection": "brick plan" if plan_info else "atomic scatter",
                    "plan": plan_info},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, args, plan_info),
                      "algorithmic_bytes_per_launch": B_dom},
         "cpu_baseline": cpu,
         "wall_s_timed": t_wall,

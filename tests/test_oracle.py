@@ -110,6 +110,57 @@ def test_traversal_matches_exact_rational(oracle, n, npix, nv, sun):
     assert checked > nv * npix * npix // 2
 
 
+@pytest.mark.parametrize("n,npix,nv,sun", [(8, 6, 5, False), (16, 5, 3, False), (8, 9, 4, True)])
+def test_traversal_fp32_matches_exact_rational(oracle, n, npix, nv, sun):
+    """The float variant, against the exact traversal of its own float-rounded
+    geometry (map borders, observer and unit vector as float32 values).  Rays
+    that pass within 1e-3 voxel of a tie, or whose sphere test is within 1e-3
+    of |r|^2 = 1, are left out: there float rounding may pick either side."""
+    from fractions import Fraction as F
+    ch = T.centered_cubic_map_header(3.0, n)
+    mrec = C.maprec(ch).astype(np.float32).astype(np.float64)
+    ih = T.centered_image_header(T.fov(ch, 4.0), npix)
+    lons = 0.37 + 2 * np.pi * np.arange(nv) / nv
+    hs = T.circular_trajectory_header(radius=4.0, longitudes=lons,
+                                      latitudes=0.2 * np.sin(np.arange(nv)), **ih)
+    recs = T.headers.view_records(hs)
+    cd = float(mrec[0])
+    checked = 0
+    for t in range(nv):
+        M = recs[t][15:18].astype(np.float32).astype(np.float64)
+        for i in range(npix):
+            for j in range(npix):
+                iv, ln, u = oracle.ray_trace(mrec, (n, n, n), recs[t], i, j, sun=sun,
+                                             dtype=np.float32)
+                if not _well_conditioned(u):
+                    continue
+                u64 = u.astype(np.float64)
+                ex_all = brute.exact_traversal(M, u64, mrec[6:9], mrec[9:12], mrec[0:3], (n, n, n))
+                if any(0 < abs(l) < F(cd) / 1000 for *_, l in ex_all):
+                    continue
+                if sun:
+                    # entry points of the exact segments: skip near-sphere decisions
+                    amin = max((F(float(mrec[6 + k] if u64[k] > 0 else mrec[9 + k])) - F(M[k]))
+                               / F(u64[k]) for k in range(3) if u64[k] != 0)
+                    near = False
+                    acc = amin
+                    for *_, l in ex_all:
+                        r2 = sum((F(M[k]) + acc * F(u64[k])) ** 2 for k in range(3))
+                        if abs(r2 - 1) < F(1, 1000):
+                            near = True
+                        acc += l
+                    if near:
+                        continue
+                ex = brute.exact_traversal(M, u64, mrec[6:9], mrec[9:12], mrec[0:3], (n, n, n),
+                                           sun=sun)
+                got = [(tuple(int(x) for x in a), float(l)) for a, l in zip(iv, ln) if abs(l) > 1e-6]
+                exs = [((a, b, c), float(l)) for a, b, c, l in ex if abs(l) > 1e-6]
+                assert [g[0] for g in got] == [e[0] for e in exs], (t, i, j)
+                np.testing.assert_allclose([g[1] for g in got], [e[1] for e in exs], atol=2e-5)
+                checked += 1
+    assert checked > nv * npix * npix // 3
+
+
 def test_axis_aligned_and_diagonal_rays_exact(oracle):
     """Rays along an axis (u components exactly 0) and through voxel corners."""
     n = 4

@@ -40,6 +40,12 @@ extern "C" {
                                 order (view, row, column), bit-exact to nthread=1 */
 #define TG_ZERO_OUT 4u       /* pj/pjt: write every pixel of the launched views as if the
                                 image stack had been zeroed first (lo_wrapper.py:26) */
+#define TG_ACC_F64 8u        /* bpj/bpjt (atomic, plan and batched): sum the per-step products
+                                d * data (rounded in the data type, as C:923-928) in the float64
+                                buffer tg_call.acc, then add each voxel's sum, rounded once, to
+                                the cube.  The result no longer depends on the order in which
+                                rays are summed (up to rare one-ulp rounding ties).  Needs a
+                                dense cube: strides (n1*n2, n2, 1) and n0*n1*n2 per time bin. */
 
 /* Error codes */
 #define TG_OK 0
@@ -90,6 +96,8 @@ typedef struct tg_call {
   int32_t *status;          /* device int32 or NULL: set to TG_EGEOMETRY if a ray stalls */
   void *workspace;          /* device scratch for TG_DETERMINISTIC */
   size_t workspace_bytes;
+  double *acc;              /* device, TG_ACC_F64: n0*n1*n2*nt doubles, zero on entry and
+                               zero again when the call's work completes */
 } tg_call;
 
 /* Size in bytes of the packed view descriptors for nv views of n1 x n2 pixels. */
@@ -152,7 +160,8 @@ int tg_scale_cube(const tg_call *c, void *stream);
  * in the atomic mode). */
 typedef struct tg_bpj_plan {
   int32_t brick[3];      /* brick edge per axis, powers of two; 0,0,0 = default (128 KiB) */
-  int32_t chunk_records; /* records per workgroup chunk; 0 = default */
+  int32_t chunk_records; /* records per workgroup chunk; 0 = chosen by tg_bpj_plan_count
+                            (about 1024 chunks of 2K..32K records) and written back */
   int32_t nb_axis[3];    /* bricks per axis (set by tg_bpj_plan_init) */
   int32_t nbricks;       /* bricks x time bins (set by tg_bpj_plan_init) */
   int32_t order;         /* 0 = records of a brick in trace order, 1 = permuted (spreads the

@@ -279,22 +279,67 @@ def test_torch_inputs_accumulate_in_place():
 
 
 # ---- brick-privatised backprojection (plan) ---------------------------------------------
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("dtype,acc64", [(np.float32, True), (np.float32, False),
+                                         (np.float64, False)])
 @pytest.mark.parametrize("obstacle", [None, "sun"])
-def test_plan_backprojection_c1(oracle, dtype, obstacle):
+def test_plan_backprojection_c1(oracle, dtype, acc64, obstacle):
     from tomograpy_amd.lo_wrapper import siddon_lo
-    print("plan bpj C1", np.dtype(dtype).name, obstacle, flush=True)
+    print("plan bpj C1", np.dtype(dtype).name, acc64, obstacle, flush=True)
     ch, cube, data = c1_problem(dtype)
     rng = np.random.default_rng(21)
     y = rng.uniform(-1, 1, data.shape).astype(dtype)
     ref = np.zeros(cube.shape, dtype)
     oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref,
                   T.InfoArray(y, header=data.header), sun=obstacle == "sun", nthread=1)
-    P = siddon_lo(data.header, ch, dtype=dtype, obstacle=obstacle, bpj_mode="plan")
+    P = siddon_lo(data.header, ch, dtype=dtype, obstacle=obstacle, bpj_mode="plan", acc64=acc64)
     out = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
     T.check_status()
     assert P._plan.records > 0
     assert rel_l2(out, ref) <= (1e-5 if dtype == np.float32 else 1e-12)
+
+
+# ---- float64 accumulation of float32 backprojections (TG_ACC_F64) ----------------------
+def assert_ulp_close(a, b, max_frac=1e-3):
+    """Equal except, in at most max_frac of the entries, by one unit in the last place."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    diff = a != b
+    if diff.any():
+        sp = np.spacing(np.maximum(np.abs(a), np.abs(b)))
+        assert np.all(np.abs(a - b)[diff] <= sp[diff]), np.abs(a - b)[diff].max()
+    assert diff.mean() <= max_frac, diff.mean()
+
+
+def test_acc64_backprojection_is_order_independent(oracle):
+    """Atomic scatter, brick plan and batched launch sum the same float32 products
+    in float64: the three cubes agree up to rare one-ulp ties, each is within
+    1e-5 of the oracle, and the accumulator is zero again afterwards."""
+    from tomograpy_amd import batch as B
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    ch, cube, data = c1_problem(np.float32)
+    rng = np.random.default_rng(23)
+    y = rng.uniform(-1, 1, data.shape).astype(np.float32)
+    ref = np.zeros(cube.shape, np.float32)
+    oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref,
+                  T.InfoArray(y, header=data.header), nthread=1)
+    yt = torch.from_numpy(y).cuda()
+    Pa = siddon_lo(data.header, ch, dtype=np.float32, bpj_mode="atomic")
+    Pp = siddon_lo(data.header, ch, dtype=np.float32, bpj_mode="plan")
+    assert Pa.acc64 and Pp.acc64
+    oa = Pa.rmatvec(yt).cpu().numpy()
+    oa2 = Pa.rmatvec(yt).cpu().numpy()
+    op = Pp.rmatvec(yt).cpu().numpy()
+    ob = torch.zeros_like(Pa.xin)
+    B.run([Pa, Pp], [yt, yt], [ob, torch.zeros_like(ob)], backward=True)
+    ob = ob.cpu().numpy()
+    T.check_status()
+    for o in (oa, op, ob):
+        assert rel_l2(o, ref) <= 1e-5
+    assert_ulp_close(oa2, oa)
+    assert_ulp_close(op, oa)
+    assert_ulp_close(ob, oa)
+    assert float(Pa.acc_buffer().abs().max()) == 0.0
+    assert float(Pp.acc_buffer().abs().max()) == 0.0
 
 
 @pytest.mark.parametrize("n,brick,chunk", [(40, (16, 16, 16), 0), (24, (8, 16, 32), 100),

@@ -117,9 +117,11 @@ def test_call_validation_errors_without_gpu():
 
 @pytest.mark.parametrize("dtype_code", [N.TG_F32, N.TG_F64])
 @pytest.mark.parametrize("schedule", [0, 1])
-def test_batch_job_table_layout(dtype_code, schedule):
-    """The staged job table has the stride and fields the persistent kernel reads
-    (entries JobDesc<T>[k] in queue order, first-tile prefix, per-job pointers)."""
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_job_table_layout(dtype_code, schedule, mixed):
+    """The staged job table has the stride and fields the persistent kernels read
+    (entries JobDesc<T>[k] in queue order, grouped by variant -- here the
+    obstacle -- with first-tile offsets counted from the group's start)."""
     rng = np.random.default_rng(0)
     njobs = 7
     calls = (N.TgCall * njobs)()
@@ -129,6 +131,7 @@ def test_batch_job_table_layout(dtype_code, schedule):
         c = calls[k]
         c.dtype = dtype_code
         c.kind = N.TG_PJ
+        c.flags = N.TG_OBSTACLE_SUN if (mixed and k % 3 == 0) else 0
         n = int(rng.integers(8, 40))
         c.map.n[0] = c.map.n[1] = c.map.n[2] = n
         c.map.nt = 1
@@ -150,19 +153,19 @@ def test_batch_job_table_layout(dtype_code, schedule):
     jb, to, co = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
     N.check(N.lib().tg_batch_job_layout(dtype_code, ctypes.byref(jb), ctypes.byref(to), ctypes.byref(co)))
     assert 256 + njobs * jb.value <= nbytes
-    order = list(range(njobs))
-    if schedule == 1:
-        order = sorted(order, key=lambda k: -sizes[k])  # stable: longest rays first
+    group = [int(calls[k].flags & N.TG_OBSTACLE_SUN) for k in range(njobs)]
+    # stable: by group, then (schedule 1) longest rays first
+    order = sorted(range(njobs), key=lambda k: (group[k], -sizes[k] if schedule == 1 else 0))
     hdr = stage[:256].view(np.uint64)
     assert hdr[1] == sum(expect_tiles) and hdr[2] == njobs
-    begin = 0
+    begin = {}
     for q, k in enumerate(order):
         base = 256 + q * jb.value
         tb = int(stage[base + to.value: base + to.value + 8].view(np.uint64)[0])
         cube = int(stage[base + co.value: base + co.value + 8].view(np.uint64)[0])
-        assert tb == begin, (q, k)
+        assert tb == begin.get(group[k], 0), (q, k)
         assert cube == calls[k].cube, (q, k)
-        begin += expect_tiles[k]
+        begin[group[k]] = begin.get(group[k], 0) + expect_tiles[k]
 
 
 @pytest.mark.parametrize("dtype_code,n,brick", [(N.TG_F32, 40, (0, 0, 0)), (N.TG_F64, 70, (0, 0, 0)),
@@ -188,7 +191,7 @@ def test_bpj_plan_init_fields(dtype_code, n, brick):
     assert list(p.nb_axis) == nb
     assert p.nbricks == nb[0] * nb[1] * nb[2] * 3
     assert p.order == 1 and p.records == 0 and p.chunks == 0
-    assert p.chunk_records == 32768
+    assert p.chunk_records == 0  # chosen by the count
     assert p.meta_bytes >= p.nbricks * (4 + 8 + 4 + 8 + 4)
     p.brick[0] = 12
     assert N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)) == N.TG_EINVAL

@@ -15,6 +15,7 @@ TG_PJ, TG_BPJ, TG_PJT, TG_BPJT = 0, 1, 2, 3
 TG_OBSTACLE_SUN = 1
 TG_DETERMINISTIC = 2
 TG_ZERO_OUT = 4
+TG_ACC_F64 = 8
 TG_HDR = 18
 TG_CG_SCAL = 16 + 1024
 
@@ -42,7 +43,8 @@ class TgCall(ctypes.Structure):
                 ("pixmask", _vp), ("mask_stride", _i64 * 3),
                 ("tindex", _vp), ("pix_weight", _vp), ("vox_weight", _vp),
                 ("voxmask", _vp), ("vox_stride", _i64 * 3),
-                ("status", _vp), ("workspace", _vp), ("workspace_bytes", _sz)]
+                ("status", _vp), ("workspace", _vp), ("workspace_bytes", _sz),
+                ("acc", _vp)]
 
 
 class TgCgSeg(ctypes.Structure):

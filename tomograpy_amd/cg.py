@@ -147,9 +147,22 @@ def iterate(solves, table, tuner=None):
         outs = [s.out for s in solves]
         plan_f = tuner.plan_for(ops, ins, outs, False, zero_out=True) if tuner else None
         B.run(ops, ins, outs, backward=False, plan=plan_f, zero_out=True)
-        qs = [s.view(s.qs) for s in solves]
-        plan_b = tuner.plan_for(ops, outs, qs, True) if tuner else None
-        B.run(ops, outs, qs, backward=True, plan=plan_b)
+        # large cubes backproject through their brick plans (an LDS-privatised
+        # launch each); the rest share one batched atomic launch
+        rest = []
+        for s in solves:
+            if s.P.use_plan():
+                _backward_one(s)
+            else:
+                rest.append(s)
+        if len(rest) == 1:
+            _backward_one(rest[0])
+        elif rest:
+            ops = [s.P for s in rest]
+            outs = [s.out for s in rest]
+            qs = [s.view(s.qs) for s in rest]
+            plan_b = tuner.plan_for(ops, outs, qs, True) if tuner else None
+            B.run(ops, outs, qs, backward=True, plan=plan_b)
     table.op(N.TG_CG_OP_REGULARIZE)
     table.op(N.TG_CG_OP_DOT_PQ)
     table.op(N.TG_CG_OP_UPDATE)

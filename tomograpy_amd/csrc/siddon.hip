@@ -208,6 +208,16 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a) {
   a.voxw = (const T *)c->vox_weight;
   a.voxmask = c->voxmask;
   a.status = c->status;
+  a.acc = nullptr;
+  if ((c->flags & TG_ACC_F64) && (c->kind == TG_BPJ || c->kind == TG_BPJT) &&
+      !(c->flags & TG_DETERMINISTIC)) {
+    if (!c->acc) return set_error(TG_EINVAL, "TG_ACC_F64 needs the acc buffer");
+    const int64_t n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
+    if (a.cs[2] != 1 || a.cs[1] != n2 || a.cs[0] != n1 * n2 ||
+        (a.nt > 1 && a.cs[3] != n0 * n1 * n2))
+      return set_error(TG_EINVAL, "TG_ACC_F64 needs a dense cube (time bin, axis 0, 1, 2 order)");
+    a.acc = c->acc;
+  }
   return TG_OK;
 }
 
@@ -233,6 +243,33 @@ void launch_siddon(const KArgs<T> &a, uint32_t blocks, hipStream_t s, bool bwd) 
   if (fast) launch_siddon_f<T, SUN, VOX, true>(a, blocks, s, bwd);
   else launch_siddon_f<T, SUN, VOX, false>(a, blocks, s, bwd);
 }
+
+// TG_ACC_F64: cube += (T)acc, acc = 0, over the dense cube of every time bin.
+template <typename T>
+__global__ __launch_bounds__(256) void k_acc_finish(T *__restrict__ cube, double *__restrict__ acc,
+                                                    int64_t total) {
+  for (int64_t k = blockIdx.x * 256ll + threadIdx.x; k < total; k += (int64_t)gridDim.x * 256) {
+    const double v = acc[k];
+    if (v != 0.0) {
+      cube[k] += (T)v;
+      acc[k] = 0.0;
+    }
+  }
+}
+
+template <typename T>
+int finish_acc(const KArgs<T> &a, hipStream_t s) {
+  if (!a.acc) return TG_OK;
+  const int64_t total = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.nt;
+  const int64_t want = (total + 255) / 256;
+  const uint32_t blocks = (uint32_t)(want < 8192 ? want : 8192);
+  if (blocks == 0) return TG_OK;
+  hipLaunchKernelGGL((k_acc_finish<T>), dim3(blocks), dim3(256), 0, s, a.cube, a.acc, total);
+  return hip_check(hipGetLastError(), "accumulator finish launch");
+}
+
+template int finish_acc<float>(const KArgs<float> &, hipStream_t);
+template int finish_acc<double>(const KArgs<double> &, hipStream_t);
 
 template <typename T>
 int run_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, hipStream_t s) {
@@ -264,7 +301,8 @@ int run_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, hipStream
     if (vox) launch_siddon<T, false, true>(a, blocks, s, bwd);
     else launch_siddon<T, false, false>(a, blocks, s, bwd);
   }
-  return hip_check(hipGetLastError(), "siddon kernel launch");
+  if ((rc = hip_check(hipGetLastError(), "siddon kernel launch"))) return rc;
+  return finish_acc<T>(a, s);
 }
 
 int dispatch_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, void *stream) {

@@ -45,36 +45,12 @@ size_t job_bytes() {
                   16);
 }
 
-template <typename T, bool BWD, bool FAST>
-__device__ __forceinline__ void run_tile_f(const KArgs<T> &a, uint32_t sun, uint32_t vox,
-                                           uint32_t tile) {
-  if (BWD) {
-    if (sun) {
-      if (vox) backproject_tile<T, true, true, FAST>(a, tile);
-      else backproject_tile<T, true, false, FAST>(a, tile);
-    } else {
-      if (vox) backproject_tile<T, false, true, FAST>(a, tile);
-      else backproject_tile<T, false, false, FAST>(a, tile);
-    }
-  } else {
-    if (sun) {
-      if (vox) project_tile<T, true, true, FAST>(a, tile);
-      else project_tile<T, true, false, FAST>(a, tile);
-    } else {
-      if (vox) project_tile<T, false, true, FAST>(a, tile);
-      else project_tile<T, false, false, FAST>(a, tile);
-    }
-  }
-}
-
-template <typename T, bool BWD>
-__device__ __forceinline__ void run_tile(const KArgs<T> &a, uint32_t sun, uint32_t vox,
-                                         uint32_t fast, uint32_t tile) {
-  if (fast) run_tile_f<T, BWD, true>(a, sun, vox, tile);
-  else run_tile_f<T, BWD, false>(a, sun, vox, tile);
-}
-
-template <typename T, bool BWD>
+// One persistent kernel per variant (obstacle, voxel operand, walk kind): a
+// single kernel dispatching all eight at run time would be register-allocated
+// for the largest (172 VGPRs in fp32, 2 waves per SIMD), while each variant
+// alone needs 26-62 and runs at full occupancy.  The host groups the jobs by
+// variant and launches one kernel per non-empty group.
+template <typename T, bool BWD, bool SUN, bool VOX, bool FAST>
 __global__ __launch_bounds__(256) void k_batch(const JobDesc<T> *__restrict__ jobs, int njobs,
                                                unsigned long long total,
                                                unsigned long long *head) {
@@ -92,12 +68,43 @@ __global__ __launch_bounds__(256) void k_batch(const JobDesc<T> *__restrict__ jo
       else hi = mid - 1;
     }
     const JobDesc<T> &jd = jobs[lo];
-    run_tile<T, BWD>(jd.a, jd.sun, jd.vox, jd.fast, (uint32_t)(tile - jd.tile_begin));
+    const uint32_t t = (uint32_t)(tile - jd.tile_begin);
+    if (BWD) backproject_tile<T, SUN, VOX, FAST>(jd.a, t);
+    else project_tile<T, SUN, VOX, FAST>(jd.a, t);
   }
 }
 
-__global__ void k_reset_head(unsigned long long *head) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *head = 0ull;
+// Header words of the device queue (64-bit): [1] tile total, [2] job count,
+// [kHeadWord + v] the tile counter of variant group v.
+constexpr int kHeadWord = 8;
+constexpr int kVariants = 8;
+
+__global__ void k_reset_heads(unsigned long long *hdr) {
+  if (threadIdx.x < kVariants && blockIdx.x == 0) hdr[kHeadWord + threadIdx.x] = 0ull;
+}
+
+template <typename T, bool BWD, bool SUN, bool VOX>
+void launch_group_f(bool fast, int blocks, hipStream_t s, const JobDesc<T> *jobs, int n,
+                    unsigned long long total, unsigned long long *head) {
+  if (fast)
+    hipLaunchKernelGGL((k_batch<T, BWD, SUN, VOX, true>), dim3(blocks), dim3(256), 0, s, jobs, n,
+                       total, head);
+  else
+    hipLaunchKernelGGL((k_batch<T, BWD, SUN, VOX, false>), dim3(blocks), dim3(256), 0, s, jobs, n,
+                       total, head);
+}
+
+template <typename T, bool BWD>
+void launch_group(int v, int blocks, hipStream_t s, const JobDesc<T> *jobs, int n,
+                  unsigned long long total, unsigned long long *head) {
+  const bool sun = v & 1, vox = v & 2, fast = v & 4;
+  if (sun) {
+    if (vox) launch_group_f<T, BWD, true, true>(fast, blocks, s, jobs, n, total, head);
+    else launch_group_f<T, BWD, true, false>(fast, blocks, s, jobs, n, total, head);
+  } else {
+    if (vox) launch_group_f<T, BWD, false, true>(fast, blocks, s, jobs, n, total, head);
+    else launch_group_f<T, BWD, false, false>(fast, blocks, s, jobs, n, total, head);
+  }
 }
 
 template <typename T>
@@ -127,31 +134,56 @@ int run_batch(const tg_call *calls, int njobs, const tg_batch_plan *plan, void *
     // longest rays first: a ray crosses at most n0 + n1 + n2 voxels
     cost[k] = (double)(cc.map.n[0] + cc.map.n[1] + cc.map.n[2]);
   }
+  // variant of each job: bit 0 obstacle, bit 1 voxel operand, bit 2 32-bit walk
+  std::vector<int> variant(njobs);
+  for (int k = 0; k < njobs; k++) {
+    const uint32_t sun = (calls[k].flags & TG_OBSTACLE_SUN) ? 1u : 0u;
+    const uint32_t vox =
+        (calls[k].voxmask != nullptr || (!bwd && calls[k].vox_weight != nullptr)) ? 1u : 0u;
+    const uint32_t fast =
+        (walk_fast_ok(args[k].n, args[k].cs) && (!vox || walk_fast_ok(args[k].n, args[k].vs)))
+            ? 1u : 0u;
+    variant[k] = (int)(sun | (vox << 1) | (fast << 2));
+  }
+  // queue order: grouped by variant; inside a group, arrival order or longest rays first
   std::vector<int> order(njobs);
   std::iota(order.begin(), order.end(), 0);
-  if (plan && plan->schedule == 1)
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  const bool longest_first = plan && plan->schedule == 1;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    if (variant[x] != variant[y]) return variant[x] < variant[y];
+    return longest_first && cost[x] > cost[y];
+  });
   char *stage = (char *)host_staging;
   memset(stage, 0, kQueueHeader);
   // The device reads the table as JobDesc<T>[njobs]: pack it with that stride.
+  // tile_begin counts from the start of the job's variant group.
   JobDesc<T> *table = (JobDesc<T> *)(stage + kQueueHeader);
-  unsigned long long begin = 0;
+  int gfirst[kVariants], gcount[kVariants];
+  unsigned long long gtiles[kVariants];
+  for (int v = 0; v < kVariants; v++) {
+    gfirst[v] = 0;
+    gcount[v] = 0;
+    gtiles[v] = 0;
+  }
+  unsigned long long total = 0;
   for (int q = 0; q < njobs; q++) {
     const int k = order[q];
+    const int v = variant[k];
+    if (gcount[v] == 0) gfirst[v] = q;
     JobDesc<T> jd;
     memset(&jd, 0, sizeof(jd));
     jd.a = args[k];
-    jd.tile_begin = begin;
-    jd.sun = (calls[k].flags & TG_OBSTACLE_SUN) ? 1u : 0u;
-    jd.vox = (calls[k].voxmask != nullptr || (!bwd && calls[k].vox_weight != nullptr)) ? 1u : 0u;
-    jd.fast = (walk_fast_ok(args[k].n, args[k].cs) && (!jd.vox || walk_fast_ok(args[k].n, args[k].vs)))
-                  ? 1u : 0u;
+    jd.tile_begin = gtiles[v];
+    jd.sun = (uint32_t)(v & 1);
+    jd.vox = (uint32_t)((v >> 1) & 1);
+    jd.fast = (uint32_t)((v >> 2) & 1);
     memcpy(&table[q], &jd, sizeof(jd));
-    begin += tiles[k];
+    gcount[v]++;
+    gtiles[v] += tiles[k];
+    total += tiles[k];
   }
-  const unsigned long long total = begin;
   const size_t bytes = kQueueHeader + (size_t)njobs * sizeof(JobDesc<T>);
-  // the header records the job count and the tile total (read by tg_batch_plan_table)
+  // the header records the tile total and the job count (read by tg_batch_plan_table)
   ((unsigned long long *)stage)[1] = total;
   ((unsigned long long *)stage)[2] = (unsigned long long)njobs;
   if (!queue) return TG_OK;  // plan only: the table is in host_staging
@@ -160,7 +192,7 @@ int run_batch(const tg_call *calls, int njobs, const tg_batch_plan *plan, void *
                       "batch job table copy")))
     return rc;
   if (total == 0) return TG_OK;
-  unsigned long long *head = (unsigned long long *)queue;
+  unsigned long long *hdr = (unsigned long long *)queue;
   const JobDesc<T> *jobs = (const JobDesc<T> *)((char *)queue + kQueueHeader);
   int blocks = plan ? plan->blocks : 0;
   if (blocks <= 0) {
@@ -171,13 +203,20 @@ int run_batch(const tg_call *calls, int njobs, const tg_batch_plan *plan, void *
       cus = 256;
     blocks = cus * 8;
   }
-  if ((unsigned long long)blocks > total) blocks = (int)total;
-  hipLaunchKernelGGL(k_reset_head, dim3(1), dim3(64), 0, s, head);
+  hipLaunchKernelGGL(k_reset_heads, dim3(1), dim3(64), 0, s, hdr);
+  for (int v = 0; v < kVariants; v++) {
+    if (gcount[v] == 0 || gtiles[v] == 0) continue;
+    const int b = (unsigned long long)blocks > gtiles[v] ? (int)gtiles[v] : blocks;
+    if (bwd)
+      launch_group<T, true>(v, b, s, jobs + gfirst[v], gcount[v], gtiles[v], hdr + kHeadWord + v);
+    else
+      launch_group<T, false>(v, b, s, jobs + gfirst[v], gcount[v], gtiles[v], hdr + kHeadWord + v);
+  }
+  if ((rc = hip_check(hipGetLastError(), "batch kernel launch"))) return rc;
   if (bwd)
-    hipLaunchKernelGGL((k_batch<T, true>), dim3(blocks), dim3(256), 0, s, jobs, njobs, total, head);
-  else
-    hipLaunchKernelGGL((k_batch<T, false>), dim3(blocks), dim3(256), 0, s, jobs, njobs, total, head);
-  return hip_check(hipGetLastError(), "batch kernel launch");
+    for (int k = 0; k < njobs; k++)
+      if ((rc = finish_acc<T>(args[k], s))) return rc;
+  return TG_OK;
 }
 
 }  // namespace

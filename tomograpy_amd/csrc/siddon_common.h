@@ -63,6 +63,7 @@ struct KArgs {
   const double *trig;
   T *cube;
   T *data;
+  double *acc;  // TG_ACC_F64: dense float64 accumulator of a backprojection, else NULL
   const uint8_t *pixmask;
   const int32_t *tindex;
   const T *pixw;

@@ -12,6 +12,11 @@ int set_error(int code, const char *fmt, ...);
 // Maps a HIP error to TG_EHIP with a message; TG_OK on success.
 int hip_check(hipError_t e, const char *what);
 
+// TG_ACC_F64: adds the float64 accumulator a.acc to the cube and clears it
+// (no-op when a.acc is NULL); enqueued after a backprojection's kernels.
+template <typename T>
+int finish_acc(const KArgs<T> &a, hipStream_t s);
+
 // Deterministic backprojection (siddon_det.hip).
 template <typename T>
 int run_deterministic(const tg_call *c, const KArgs<T> &a, hipStream_t s);

@@ -264,15 +264,17 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
     }
   }
   __syncthreads();
-  T *cube = a.cube + (int64_t)bin * a.cs[3];
+  const int64_t boff = (int64_t)bin * a.cs[3];
   for (int k = threadIdx.x; k < vol; k += blockDim.x) {
     const Acc sum = acc[k];
     if (sum == (Acc)0) continue;
-    const T v = (T)sum;
     const int l2 = k & (g.b[2] - 1), l1 = (k >> g.lg[2]) & (g.b[1] - 1), l0 = k >> (g.lg[2] + g.lg[1]);
     const int64_t i0 = lo0 + l0, i1 = lo1 + l1, i2 = lo2 + l2;
     if (a.voxmask && a.voxmask[i0 * a.vs[0] + i1 * a.vs[1] + i2 * a.vs[2]]) continue;
-    unsafeAtomicAdd(cube + i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2], v);
+    const int64_t off = boff + i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2];
+    // TG_ACC_F64 keeps the brick's partial sum in float64 until the finish pass
+    if (a.acc) unsafeAtomicAdd(a.acc + off, (double)sum);
+    else unsafeAtomicAdd(a.cube + off, (T)sum);
   }
 }
 
@@ -306,6 +308,10 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
   g.shuffle = p->order == 1 ? 1 : 0;
   return TG_OK;
 }
+
+// Target number of chunks (one 1024-lane workgroup each, one resident per CU
+// with 128 KiB of LDS accumulators) when the caller leaves the chunk size open.
+constexpr unsigned long long kAutoChunks = 1024;
 
 struct MetaLayout {
   size_t counts, offsets, cursor, chunk_prefix, mult, total;
@@ -375,8 +381,18 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
                                      hipMemcpyDeviceToHost, s), "plan counts copy")))
     return rc;
   if ((rc = hip_check(hipStreamSynchronize(s), "plan count sync"))) return rc;
-  std::vector<unsigned long long> off(g.nbricks + 1), cp(g.nbricks + 1);
   unsigned long long tot = 0, ch = 0;
+  for (int b = 0; b < g.nbricks; b++) tot += hc[b];
+  if (p->chunk_records <= 0) {
+    // automatic: about kAutoChunks workgroups, each with 2K..32K records
+    unsigned long long want = (tot + kAutoChunks - 1) / kAutoChunks;
+    int chunk = 2048;
+    while (chunk < 32768 && (unsigned long long)chunk < want) chunk <<= 1;
+    g.chunk = chunk;
+    p->chunk_records = chunk;
+  }
+  std::vector<unsigned long long> off(g.nbricks + 1), cp(g.nbricks + 1);
+  tot = 0;
   for (int b = 0; b < g.nbricks; b++) {
     off[b] = tot;
     cp[b] = ch;
@@ -483,8 +499,10 @@ int plan_bpj(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   const size_t lds = sizeof(Acc) * (size_t)g.b[0] * g.b[1] * g.b[2];
   if (lds > 160 * 1024) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
   if (p->chunks == 0) return TG_OK;
-  if (c->flags & TG_OBSTACLE_SUN) return launch_plan_bpj<T, true>(a, g, p, s);
-  return launch_plan_bpj<T, false>(a, g, p, s);
+  rc = (c->flags & TG_OBSTACLE_SUN) ? launch_plan_bpj<T, true>(a, g, p, s)
+                                    : launch_plan_bpj<T, false>(a, g, p, s);
+  if (rc) return rc;
+  return finish_acc<T>(a, s);
 }
 
 }  // namespace
@@ -505,7 +523,6 @@ int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p) {
   BrickGeom g;
   int rc = make_geom(c, p, g);
   if (rc) return rc;
-  p->chunk_records = g.chunk;
   p->nbricks = g.nbricks;
   for (int k = 0; k < 3; k++) p->nb_axis[k] = g.nb[k];
   p->meta_bytes = meta_layout(g.nbricks).total;

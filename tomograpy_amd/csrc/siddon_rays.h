@@ -68,6 +68,31 @@ __device__ __forceinline__ void atomic_add(T *p, T v) {
   unsafeAtomicAdd(p, v);
 }
 
+// Walks one ray and adds the reference's per-step product d * w (rounded in T,
+// C:923-928) to dst[voxel]: dst is the cube itself, or the float64
+// accumulator of TG_ACC_F64 (same element offsets; the cube is then dense).
+template <typename T, bool SUN, bool VOX, bool FAST, typename A>
+__device__ __forceinline__ void scatter_ray(const KArgs<T> &a, const ViewHdr<T> &h, const T u0[3],
+                                            Ray<T> &r, T w, A *dst) {
+  Walk<FAST> wk, wm;
+  walk_init<FAST>(r, a.n, a.cs, wk);
+  if (VOX) walk_init<FAST>(r, a.n, a.vs, wm);
+  int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
+  while (walk_inside(wk) && (!SUN || outside_sun(h.M, u0, r.ac))) {
+    if (--guard < 0) {
+      flag_status(a.status, TG_EGEOMETRY);
+      break;
+    }
+    const auto lin = wk.off();
+    bool keep = true;
+    if (VOX) {
+      if (a.voxmask) keep = (a.voxmask[wm.off()] == 0);
+    }
+    const T d = step_walk<T, SUN, FAST, VOX>(r, wk, wm);
+    if (keep) atomic_add(dst + lin, (A)(d * w));
+  }
+}
+
 // Backprojection (bpj / bpjt) as an atomic scatter.
 template <typename T, bool SUN, bool VOX, bool FAST>
 __device__ __forceinline__ void backproject_tile(const KArgs<T> &a, uint32_t tile) {
@@ -92,24 +117,10 @@ __device__ __forceinline__ void backproject_tile(const KArgs<T> &a, uint32_t til
   int64_t toff;
   if (!time_offset(a, t, toff)) return;
   if (!in_map(a, r)) return;
-  T *cube = a.cube + toff;
-  Walk<FAST> wk, wm;
-  walk_init<FAST>(r, a.n, a.cs, wk);
-  if (VOX) walk_init<FAST>(r, a.n, a.vs, wm);
-  int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
-  while (walk_inside(wk) && (!SUN || outside_sun(h.M, u0, r.ac))) {
-    if (--guard < 0) {
-      flag_status(a.status, TG_EGEOMETRY);
-      break;
-    }
-    const auto lin = wk.off();
-    bool keep = true;
-    if (VOX) {
-      if (a.voxmask) keep = (a.voxmask[wm.off()] == 0);
-    }
-    const T d = step_walk<T, SUN, FAST, VOX>(r, wk, wm);
-    if (keep) atomic_add(cube + lin, d * w);
-  }
+  if (a.acc)
+    scatter_ray<T, SUN, VOX, FAST>(a, h, u0, r, w, a.acc + toff);
+  else
+    scatter_ray<T, SUN, VOX, FAST>(a, h, u0, r, w, a.cube + toff);
 }
 
 }  // namespace tg

@@ -72,7 +72,7 @@ class Siddon:
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
                  voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
                  tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0,
-                 plan_order=0):
+                 plan_order=0, acc64=None):
         self.data_header = list(data_header)
         self.map_header = dict(cube_header)
         check_headers(self.data_header, self.map_header)
@@ -122,15 +122,64 @@ class Siddon:
         self.plan_order = int(plan_order)
         self._plan = None
         self._plan_bufs = None
+        self._plan_count = None
+        self._auto_plan = None
+        # float32 backprojections sum in a float64 accumulator (TG_ACC_F64), so
+        # that the result does not depend on the order in which rays are added
+        self.acc64 = (self.tdtype == torch.float32) if acc64 is None else bool(acc64)
+        self._acc = None
 
     # -- brick-privatised backprojection plan ------------------------------
+    # In "auto" mode the plan is used when an atomic scatter would update each
+    # voxel (of every time bin) more than this many times on average, and the
+    # geometry has at least PLAN_MIN_STEPS steps: below that the scatter's
+    # contention is low and the plan's build and per-chunk flush do not pay.
+    PLAN_MIN_STEPS_PER_VOXEL = 16
+    PLAN_MIN_STEPS = 1 << 20
+
     def use_plan(self):
         """Whether backprojections go through the brick plan."""
         if self.deterministic or self.bpj_mode == "atomic":
             return False
         if self.bpj_mode == "plan":
             return True
-        return int(np.prod(self.cube_shape)) >= 64 ** 3 and max(self.cube_shape) <= 1024
+        if max(self.cube_shape) > 1024:
+            return False
+        if self._auto_plan is None:
+            steps = self.count_steps()
+            voxels = int(np.prod(self.cube_shape)) * getattr(self, "nt", 1)
+            self._auto_plan = (steps >= self.PLAN_MIN_STEPS and
+                               steps >= self.PLAN_MIN_STEPS_PER_VOXEL * voxels)
+        return self._auto_plan
+
+    def _count_plan(self):
+        """Counts the plan's records (synchronises); returns (plan, meta buffer)."""
+        if self._plan_count is None:
+            lib = N.lib()
+            c = self._call(N.TG_BPJ, self.xin, self.xout)
+            p = N.TgBpjPlan()
+            if self.brick is not None:
+                for k in range(3):
+                    p.brick[k] = int(self.brick[k])
+            p.chunk_records = self.chunk_records
+            p.order = self.plan_order
+            N.check(lib.tg_bpj_plan_init(ctypes.byref(c), ctypes.byref(p)))
+            meta = torch.empty(max(int(p.meta_bytes), 1), dtype=torch.uint8, device=self.dev)
+            p.meta = meta.data_ptr()
+            p.meta_bytes = meta.numel()
+            N.check(lib.tg_bpj_plan_count(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
+            self._plan_count = (p, meta)
+        return self._plan_count
+
+    def plan_reserve_bytes(self):
+        """Device bytes the brick plan of this geometry takes (0 without a plan);
+        counts the plan, which synchronises the stream, but allocates no records."""
+        if not self.use_plan():
+            return 0
+        if self._plan_bufs is not None:
+            return self.plan_bytes()
+        p, meta = self._count_plan()
+        return int(meta.numel()) + max(int(p.rec_bytes), 1)
 
     def build_plan(self):
         """Builds (once) the value-independent brick plan of this geometry."""
@@ -138,17 +187,8 @@ class Siddon:
             return self._plan
         lib = N.lib()
         c = self._call(N.TG_BPJ, self.xin, self.xout)
-        p = N.TgBpjPlan()
-        if self.brick is not None:
-            for k in range(3):
-                p.brick[k] = int(self.brick[k])
-        p.chunk_records = self.chunk_records
-        p.order = self.plan_order
-        N.check(lib.tg_bpj_plan_init(ctypes.byref(c), ctypes.byref(p)))
-        meta = torch.empty(max(int(p.meta_bytes), 1), dtype=torch.uint8, device=self.dev)
-        p.meta = meta.data_ptr()
-        p.meta_bytes = meta.numel()
-        N.check(lib.tg_bpj_plan_count(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
+        p, meta = self._count_plan()
+        self._plan_count = None
         rec = torch.empty(max(int(p.rec_bytes), 1), dtype=torch.uint8, device=self.dev)
         p.rec = rec.data_ptr()
         p.rec_bytes = rec.numel()
@@ -175,7 +215,23 @@ class Siddon:
                 self._ws = S.det_workspace(c, self.dev)
             c.workspace = self._ws.data_ptr()
             c.workspace_bytes = self._ws.numel()
+        elif kind in (N.TG_BPJ, N.TG_BPJT) and self.acc64 and self._dense(cube):
+            c.flags |= N.TG_ACC_F64
+            c.acc = self.acc_buffer().data_ptr()
         return c
+
+    def _dense(self, cube):
+        """Whether `cube` has the dense layout TG_ACC_F64 needs."""
+        n0, n1, n2 = self.cube_shape
+        st = tuple(cube.stride())
+        return st[:3] == (n1 * n2, n2, 1) and (len(st) == 3 or st[3] == n0 * n1 * n2)
+
+    def acc_buffer(self):
+        """The float64 backprojection accumulator (zero between calls)."""
+        if self._acc is None:
+            n = int(np.prod(self.cube_shape)) * getattr(self, "nt", 1)
+            self._acc = torch.zeros(n, dtype=torch.float64, device=self.dev)
+        return self._acc
 
     def _as_cube(self, x):
         x = torch.as_tensor(x, device=self.dev) if not isinstance(x, torch.Tensor) else x

@@ -1,12 +1,13 @@
 """Continuous-batching reconstruction server.
 
 Requests (an image stack with its headers, and CG settings) arrive over time.
-The server admits a waiting request as soon as its HBM footprint fits the
-budget and a slot is free, keeps its cube, images and CG state on the device,
-and advances every admitted request by one CG iteration per engine step: one
-batched projector launch and one batched backprojector launch over all of
-them (``tg_batch_run``), plus the segmented CG vector kernels
-(``tg_cg_batch``).  A request leaves when its relative residual reaches its
+The server admits a waiting request as soon as its HBM footprint (CG
+vectors, images and, for a large cube, its brick plan) fits the budget and a
+slot is free, keeps its cube, images and CG state on the device, and
+advances every admitted request by one CG iteration per engine step: one
+batched projector launch over all of them (``tg_batch_run``), the brick-plan
+backprojection of each large cube, one batched backprojector launch over the
+others, and the segmented CG vector kernels (``tg_cg_batch``).  A request leaves when its relative residual reaches its
 tolerance (checked every ``check_every`` of its iterations) or at maxiter;
 requests that arrive meanwhile join at the next engine step.
 
@@ -48,9 +49,12 @@ class Request:
         self.residual = None
         self.t_admit = None
         self.t_done = None
+        self._op = None
+        self._reserved = 0
 
     def footprint(self):
-        """Device bytes while admitted: 4 CG vectors + 2 operator buffers + images."""
+        """Device bytes while admitted, without the brick plan: 4 CG vectors, 2
+        operator buffers, the float64 backprojection accumulator, images."""
         n = np.prod([self.cube_header["NAXIS%d" % (k + 1)] for k in range(3)])
         nt = 1 if self.tindex is None else int(np.max(self.tindex)) + 1
         n1 = self.data_header[0]["NAXIS1"]
@@ -58,7 +62,9 @@ class Request:
         pix = n1 * n2 * len(self.data_header)
         b = self.dtype.itemsize
         trig = len(self.data_header) * (n1 + n2) * 16
-        return int((6 * n * nt + 2 * pix) * b + trig + (pix if self.mask is not None else 0) + (1 << 16))
+        acc = 8 * n * nt if self.dtype == np.float32 else 0
+        return int((6 * n * nt + 2 * pix) * b + acc + trig + (pix if self.mask is not None else 0)
+                   + (1 << 16))
 
     def operator(self):
         if self.tindex is not None:
@@ -166,11 +172,18 @@ class Server:
                 fp = req.footprint()
                 if fp > self.budget:
                     raise ValueError("request %s needs %d bytes, budget %d" % (req.id, fp, self.budget))
+                if req._op is None:
+                    req._op = req.operator()
+                P = req._op
+                fp += P.plan_reserve_bytes()
+                req._reserved = fp
+                if fp > self.budget:
+                    raise ValueError("request %s needs %d bytes, budget %d" % (req.id, fp, self.budget))
                 if used + fp > self.budget:
                     break
                 waiting.popleft()
                 used += fp
-                P = req.operator()
+                req._op = None
                 row = self.free_rows.pop()
                 s = CG.CGSolve(P, _y_device(req, P), lam=req.lam, tol=req.tol, maxiter=req.maxiter,
                                scal=self.scal_pool[row])
@@ -198,7 +211,7 @@ class Server:
                         res = s.residual(scal[row])
                         if res <= r.tol or s.iterations >= r.maxiter:
                             _finish(r, s, t0)
-                            used -= r.footprint()
+                            used -= r._reserved
                             self.free_rows.append(row)
                             continue
                     keep.append((r, s, row))

@@ -14,7 +14,11 @@ launch time (HIP events on the launch stream), against 8 TB/s HBM.
   forward  bytes = S*b + R_t*b + V*18*b   (voxel gather per step, pixel write per ray)
   backward bytes = 2*S*b + R_t*b + V*18*b (voxel read-modify-write per step, pixel read per ray)
 cpu_baseline: the oracle (C/OpenMP restatement of the reference) on a bounded
-sample of the same workload's views, on the host cores.
+sample of the same workload's views, on the host cores; "parity" compares the
+GPU pair with the oracle's output on that sample.
+serving: the mixed-size reconstruction stream (see serving_leg), served by
+the continuous-batching server and solved one request at a time: p50/p99
+latency, throughput, and the largest relative L2 between the two results.
 """
 import argparse
 import json
@@ -45,7 +49,47 @@ def parse():
     ap.add_argument("--cg-iters", type=int, default=0, help="also time this many CG iterations")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--serve-requests", type=int, default=32,
+                    help="reconstruction stream served vs plain (0 = skip)")
+    ap.add_argument("--serve-max-active", type=int, default=8)
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="Poisson arrival rate in requests/s (0 = all requests at once)")
     return ap.parse_args()
+
+
+def serving_leg(args):
+    """The mixed-size reconstruction stream (32^3-256^3 cubes, 8-128 views of
+    64^2-512^2, pixel mask on/off, 1-8 time bins, CG to a relative residual of
+    1e-4), solved once by the continuous-batching server and once one request
+    at a time in arrival order; same requests, same seeds."""
+    import torch
+    from tomograpy_amd import serve as SV
+    from tomograpy_amd import workloads as W
+
+    def requests():
+        rng = np.random.default_rng(W.SEED0 + 5)
+        mix = W.config5_mix(args.serve_requests, rng)
+        return SV.make_requests(mix, rng, rate=args.serve_rate or None, tol=1e-4, maxiter=200)
+
+    plain_reqs = requests()
+    served_reqs = requests()
+    torch.cuda.synchronize()
+    print("serving: %d requests made" % len(plain_reqs), file=sys.stderr, flush=True)
+    plain = SV.serve_plain(plain_reqs)
+    print("serving: plain done in %.1f s" % plain["makespan_s"], file=sys.stderr, flush=True)
+    served = SV.Server(budget_bytes=128 << 30, max_active=args.serve_max_active).run(served_reqs)
+    print("serving: served done in %.1f s" % served["makespan_s"], file=sys.stderr, flush=True)
+    err = max(float(np.linalg.norm(a.x - b.x) / max(np.linalg.norm(b.x), 1e-300))
+              for a, b in zip(served_reqs, plain_reqs))
+    return {"requests": len(plain_reqs), "arrival_rate_rps": args.serve_rate or None,
+            "max_active": args.serve_max_active, "tol": 1e-4,
+            "served": served, "plain": plain,
+            "throughput_ratio": served["throughput_rps"] / plain["throughput_rps"],
+            "p50_ratio": plain["p50_s"] / served["p50_s"],
+            "max_rel_l2_served_vs_plain": err,
+            "same_iterations": all(a.iterations == b.iterations
+                                   for a, b in zip(served_reqs, plain_reqs)),
+            "all_converged": all(r.residual <= 1e-4 for r in served_reqs + plain_reqs)}
 
 
 def measured_traffic(kernel, args, plan_info):
@@ -73,8 +117,8 @@ def cpu_baseline(ch, hs, cube_np, data_np, target_s):
     mrec = map_record(ch)
     threads = O.num_threads()
     nv = len(hs)
-    # calibrate on 2 views, then size the sample to about target_s of CPU time
-    k = 2
+    # calibrate on one view per thread, then size the sample to about target_s of CPU time
+    k = max(1, min(threads, nv))
     t0 = time.perf_counter()
     d = np.zeros_like(data_np)
     c = np.zeros_like(cube_np)
@@ -86,14 +130,31 @@ def cpu_baseline(ch, hs, cube_np, data_np, target_s):
     # OpenMP splits views over threads: use a multiple of the thread count when possible
     if k2 >= threads:
         k2 = (k2 // threads) * threads
+    d[:] = 0
+    c[:] = 0
     t0 = time.perf_counter()
     s1 = O.siddon(O.PJ, recs, mrec, cube_np, d, t0=0, t1=k2, nthread=threads)
     s2 = O.siddon(O.BPJ, recs, mrec, c, data_np, t0=0, t1=k2, nthread=threads)
     dt = time.perf_counter() - t0
-    return {"value": (s1 + s2) / dt, "unit": "Siddon steps/s", "cores": threads,
-            "kind": "port",
-            "sample": "oracle pj+bpj on views 0..%d of the workload (%d of %d), %.1f s" %
-                      (k2 - 1, k2, nv, dt)}
+    cpu = {"value": (s1 + s2) / dt, "unit": "Siddon steps/s", "cores": threads,
+           "kind": "port",
+           "sample": "oracle pj+bpj on views 0..%d of the workload (%d of %d), %.1f s" %
+                     (k2 - 1, k2, nv, dt)}
+    return cpu, d[..., :k2], c, k2
+
+
+def sample_parity(ch, hs, dtype, cube_np, data_np, d_ref, c_ref, k):
+    """The GPU pair on the CPU baseline's sample of views against the oracle's
+    output for it: forward bit-exact, backward relative L2."""
+    import torch
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    Ps = siddon_lo(hs[:k], ch, dtype=dtype)
+    fwd = Ps.matvec(torch.from_numpy(cube_np).cuda()).cpu().numpy()
+    bwd = Ps.rmatvec(torch.from_numpy(np.ascontiguousarray(data_np[..., :k])).cuda()).cpu().numpy()
+    c64 = c_ref.astype(np.float64)
+    return {"views": k, "forward_bit_exact": bool(np.array_equal(fwd, d_ref)),
+            "backward_rel_l2": float(np.linalg.norm(bwd - c64) / np.linalg.norm(c64)),
+            "tolerance": 1e-5 if dtype == np.float32 else 1e-12}
 
 
 def main():
@@ -204,15 +265,21 @@ def main():
         extra["cg"] = {"iterations": info["iterations"], "ms_per_iteration":
                        e0.elapsed_time(e1) / info["iterations"], "residual": info["residual"]}
 
+    y_host = y.cpu().numpy() if not args.no_cpu else None
+    if args.serve_requests:
+        # release the pair's operator (its brick plan is 10.7 GB at 256^3) first
+        del P, x, x_true, y, noise
+        torch.cuda.empty_cache()
+        extra["serving"] = serving_leg(args)
     cpu = None
     if not args.no_cpu:
-        data_np = np.zeros((args.pix, args.pix, V), dtype=dtype)
-        y_np = y.cpu().numpy()
-        cpu = cpu_baseline(ch, hs, phantom, np.ascontiguousarray(y_np), args.cpu_seconds)
+        y_host = np.ascontiguousarray(y_host)
+        cpu, d_ref, c_ref, k = cpu_baseline(ch, hs, phantom, y_host, args.cpu_seconds)
+        extra["parity"] = sample_parity(ch, hs, dtype, phantom, y_host, d_ref, c_ref, k)
 
     value = 2 * S / (ms_step * 1e-3)
     out = {
-        "metric": "Siddon steps/s (projector+backprojector pair)",
+        "metric": "Siddon steps/s + recon p50/p99 latency, served vs plain, at <=1e-5 rel err",
         "value": value,
         "unit": "Siddon steps/s",
         "n_gpus": 1,

@@ -66,6 +66,13 @@ class Request:
         return int((6 * n * nt + 2 * pix) * b + acc + trig + (pix if self.mask is not None else 0)
                    + (1 << 16))
 
+    def work(self):
+        """Relative cost of one CG iteration: rays x mean cube edge."""
+        n = sum(self.cube_header["NAXIS%d" % (k + 1)] for k in range(3)) / 3.0
+        n1 = self.data_header[0]["NAXIS1"]
+        n2 = self.data_header[0]["NAXIS2"]
+        return float(n1 * n2 * len(self.data_header) * n)
+
     def operator(self):
         if self.tindex is not None:
             return Siddon4d(self.data_header, self.cube_header, tindex=self.tindex, mask=self.mask,
@@ -143,7 +150,14 @@ def summarize(requests, t_first, t_end):
 class Server:
     """Continuous batching over CG reconstructions on one GPU."""
 
-    def __init__(self, budget_bytes=64 << 30, max_active=64, check_every=5, tuner=None):
+    def __init__(self, budget_bytes=64 << 30, max_active=64, check_every=5, tuner=None,
+                 order="shortest"):
+        """order: "shortest" admits the waiting request with the least work first
+        (traced steps per iteration, estimated from the geometry); "arrival" admits
+        in arrival order."""
+        if order not in ("shortest", "arrival"):
+            raise ValueError("order must be 'shortest' or 'arrival'")
+        self.order = order
         self.budget = int(budget_bytes)
         self.max_active = int(max_active)
         self.check_every = int(check_every)
@@ -167,6 +181,8 @@ class Server:
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
             admitted = []
+            if self.order == "shortest" and len(waiting) > 1:
+                waiting = collections.deque(sorted(waiting, key=lambda r: r.work()))
             while waiting and len(active) < self.max_active:
                 req = waiting[0]
                 fp = req.footprint()

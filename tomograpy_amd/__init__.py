@@ -16,5 +16,7 @@ from .siddon import (
     conic_image_backprojector, ray_projector, check_projector_inputs, C_full_unit_vector,
     C_full_intersection_parameters, count_steps, check_status,
 )
+from . import solar, srt
+from .solar import read_data, update_header
 
 version = "0.1.0"

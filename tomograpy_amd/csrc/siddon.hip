@@ -208,6 +208,7 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a) {
   a.voxw = (const T *)c->vox_weight;
   a.voxmask = c->voxmask;
   a.status = c->status;
+  a.cube_bytes = buffer_walk_bytes(a.n, a.cs, (int)sizeof(T));
   a.acc = nullptr;
   if ((c->flags & TG_ACC_F64) && (c->kind == TG_BPJ || c->kind == TG_BPJT) &&
       !(c->flags & TG_DETERMINISTIC)) {

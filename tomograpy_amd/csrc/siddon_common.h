@@ -64,6 +64,9 @@ struct KArgs {
   T *cube;
   T *data;
   double *acc;  // TG_ACC_F64: dense float64 accumulator of a backprojection, else NULL
+  // bytes of one time bin of the cube when the forward projector may read it
+  // through a buffer resource (non-negative strides, byte offsets < 2^31), else 0
+  uint32_t cube_bytes;
   const uint8_t *pixmask;
   const int32_t *tindex;
   const T *pixw;
@@ -279,16 +282,16 @@ struct Walk<false> {
 // an array with element strides st.
 template <bool FAST, typename T, typename S>
 __device__ __forceinline__ void walk_init(const Ray<T> &r, const uint32_t n[3], const S st[3],
-                                          Walk<FAST> &w) {
+                                          Walk<FAST> &w, int scale = 1) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     w.r[k] = r.up[k] > 0 ? (int)n[k] - r.iv[k] : (r.up[k] < 0 ? r.iv[k] + 1 : kNoStep);
-    w.m[k] = -(r.up[k] * st[k]);
+    w.m[k] = -(r.up[k] * st[k] * scale);
   }
   if (FAST) {
     uint32_t lin = 0;
 #pragma unroll
-    for (int k = 0; k < 3; k++) lin += (uint32_t)r.iv[k] * (uint32_t)st[k];
+    for (int k = 0; k < 3; k++) lin += (uint32_t)r.iv[k] * (uint32_t)(st[k] * scale);
     w.base = lin - ((uint32_t)(w.r[0] * w.m[0]) + (uint32_t)(w.r[1] * w.m[1]) +
                     (uint32_t)(w.r[2] * w.m[2]));
   } else {
@@ -306,15 +309,16 @@ __device__ __forceinline__ bool walk_inside(const Walk<FAST> &w) {
 
 // One step of C:811-912 on (D, ac) with the counters of up to two walks
 // (cube and voxel operands) instead of iv.  Same floating-point operations as
-// step(): the minimum is taken with min3 (exact: no NaN reaches here), the
-// length is D of the first stepping axis in x, y, z order.
+// step(): the minimum is taken with min3 (exact: no NaN reaches here).  The
+// reference's length is D of the first axis holding the minimum; the minimum
+// itself is that value, up to the sign of a zero (-0 == +0), and a zero
+// length adds a zero product that leaves any nonzero sum unchanged.
 template <typename T, bool AC, bool FAST, bool TWO>
 __device__ __forceinline__ T step_walk(Ray<T> &r, Walk<FAST> &w, Walk<FAST> &w2) {
-  const T dmin = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
-  const bool sx = (r.D[0] == dmin);
-  const bool sy = (r.D[1] == dmin);
-  const bool sz = (r.D[2] == dmin);
-  const T d = sx ? r.D[0] : (sy ? r.D[1] : r.D[2]);
+  const T d = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
+  const bool sx = (r.D[0] == d);
+  const bool sy = (r.D[1] == d);
+  const bool sz = (r.D[2] == d);
   if (AC) r.ac += d;
   r.D[0] = sx ? r.pabs[0] : r.D[0] - d;
   r.D[1] = sy ? r.pabs[1] : r.D[1] - d;
@@ -328,6 +332,20 @@ __device__ __forceinline__ T step_walk(Ray<T> &r, Walk<FAST> &w, Walk<FAST> &w2)
     w2.r[2] -= sz;
   }
   return d;
+}
+
+// Whether the forward projector may walk the cube in bytes through a buffer
+// resource: non-negative strides, every stride in bytes < 2^23 (24-bit
+// multiplies) and the 3-D extent in bytes < 2^31.  Returns the extent in
+// bytes, or 0.
+inline uint32_t buffer_walk_bytes(const uint32_t n[3], const int64_t st[3], int elem) {
+  int64_t ext = 1;
+  for (int k = 0; k < 3; k++) {
+    if (st[k] < 0 || st[k] * elem >= (1 << 23)) return 0;
+    ext += (int64_t)(n[k] - 1) * st[k];
+  }
+  ext *= elem;
+  return ext < (1ll << 31) ? (uint32_t)ext : 0u;
 }
 
 // Whether the 32-bit walk applies: every |stride| < 2^23 (24-bit multiplies)

@@ -8,6 +8,23 @@
 
 namespace tg {
 
+// Descriptor word 3 of a raw buffer resource on the gfx9 family (32-bit data
+// format, no swizzle): loads past num_records return 0.
+constexpr int kBufferWord3 = 0x00020000;
+
+template <typename T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t rs, uint32_t byte_off);
+
+template <>
+__device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t rs, uint32_t byte_off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)byte_off, 0, 0));
+}
+
+template <>
+__device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t rs, uint32_t byte_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)byte_off, 0, 0));
+}
+
 // Forward projection (pj / pjt), optionally with the voxel operands of
 // O.P.M (weight, mask) and the pixel weight O.
 template <typename T, bool SUN, bool VOX, bool FAST>
@@ -34,7 +51,56 @@ __device__ __forceinline__ void project_tile(const KArgs<T> &a, uint32_t tile) {
   int64_t toff;
   if (!time_offset(a, t, toff)) return;
   T acc = (a.pixw || zero_out) ? (T)0 : a.data[did];
-  if (in_map(a, r)) {
+  if (FAST && a.cube_bytes && in_map(a, r)) {
+    // Byte walk over a buffer resource, one voxel ahead: the load of the next
+    // voxel is in flight while this step's cascade runs.  After the last step
+    // the walk may name a voxel outside the cube; the resource's range check
+    // returns 0 for it and the value is never used.
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(a.cube + toff), (short)0, (int)a.cube_bytes,
+                                          kBufferWord3);
+    Walk<FAST> w, wm;
+    walk_init<FAST>(r, a.n, a.cs, w, (int)sizeof(T));
+    if (VOX) walk_init<FAST>(r, a.n, a.vs, wm);
+    // Unrolled by two so that the two values alternate registers: no copy of a
+    // value whose load is still in flight.  Every step lowers at least one
+    // counter (the ray state is finite), so n0 + n1 + n2 steps end any ray;
+    // the wave-uniform pair counter is the guard against a stalled walk.
+    int pairs = (int)((a.n[0] + a.n[1] + a.n[2]) >> 1) + 2;
+    T v0 = buffer_load<T>(rs, w.off()), v1 = (T)0;
+    while (true) {
+      if (!(walk_inside(w) && (!SUN || outside_sun(h.M, u0, r.ac)))) break;
+      {
+        T cur = v0;
+        bool keep = true;
+        if (VOX) {
+          const auto lm = wm.off();
+          if (a.voxw) cur = a.voxw[lm] * cur;
+          if (a.voxmask) keep = (a.voxmask[lm] == 0);
+        }
+        const T d = step_walk<T, SUN, FAST, VOX>(r, w, wm);
+        v1 = buffer_load<T>(rs, w.off());
+        if (keep) acc = acc + d * cur;
+      }
+      if (!(walk_inside(w) && (!SUN || outside_sun(h.M, u0, r.ac)))) break;
+      {
+        T cur = v1;
+        bool keep = true;
+        if (VOX) {
+          const auto lm = wm.off();
+          if (a.voxw) cur = a.voxw[lm] * cur;
+          if (a.voxmask) keep = (a.voxmask[lm] == 0);
+        }
+        const T d = step_walk<T, SUN, FAST, VOX>(r, w, wm);
+        v0 = buffer_load<T>(rs, w.off());
+        if (keep) acc = acc + d * cur;
+      }
+      if (--pairs <= 0) {
+        if (walk_inside(w)) flag_status(a.status, TG_EGEOMETRY);
+        break;
+      }
+    }
+  } else if (in_map(a, r)) {
     const T *cube = a.cube + toff;
     Walk<FAST> w, wm;
     walk_init<FAST>(r, a.n, a.cs, w);

@@ -209,6 +209,16 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a) {
   a.voxmask = c->voxmask;
   a.status = c->status;
   a.cube_bytes = buffer_walk_bytes(a.n, a.cs, (int)sizeof(T));
+  {
+    const int64_t dn[3] = {c->n1, c->n2, c->nv};
+    int64_t ext = 0;
+    bool ok = true;
+    for (int k = 0; k < 3; k++) {
+      if (a.ds[k] < 0 || a.ds[k] >= (1ll << 31)) ok = false;
+      ext += (dn[k] > 0 ? dn[k] - 1 : 0) * a.ds[k];
+    }
+    a.data32 = (ok && ext < (1ll << 31)) ? 1u : 0u;
+  }
   a.acc = nullptr;
   if ((c->flags & TG_ACC_F64) && (c->kind == TG_BPJ || c->kind == TG_BPJT) &&
       !(c->flags & TG_DETERMINISTIC)) {

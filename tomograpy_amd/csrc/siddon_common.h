@@ -67,6 +67,9 @@ struct KArgs {
   // bytes of one time bin of the cube when the forward projector may read it
   // through a buffer resource (non-negative strides, byte offsets < 2^31), else 0
   uint32_t cube_bytes;
+  // nonzero when every data (and pixel-weight) offset fits 31 bits with
+  // non-negative strides: pixel offsets may use 32-bit arithmetic
+  uint32_t data32;
   const uint8_t *pixmask;
   const int32_t *tindex;
   const T *pixw;
@@ -257,6 +260,9 @@ __device__ __forceinline__ double tmin(double a, double b) { return fmin(a, b); 
 template <bool FAST>
 struct Walk;
 
+// 32-bit walk: r[k] = voxel crossings left on axis k (the ray is inside
+// while every counter is positive); offset = base + sum r[k] m[k] with
+// 24-bit multiplies.
 template <>
 struct Walk<true> {
   int r[3];
@@ -280,24 +286,41 @@ struct Walk<false> {
 
 // Counters of a ray whose voxel iv is inside [0, n) (check in_map first), for
 // an array with element strides st.
+// Sets a 32-bit walk from the crossings left per axis (each >= 1), the byte or
+// element multipliers and the offset of the current voxel.
+__device__ __forceinline__ void walk_set(Walk<true> &w, const int cnt[3], const int m[3],
+                                         uint32_t lin) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    w.r[k] = cnt[k];
+    w.m[k] = m[k];
+  }
+  w.base = lin - ((uint32_t)(w.r[0] * w.m[0]) + (uint32_t)(w.r[1] * w.m[1]) +
+                  (uint32_t)(w.r[2] * w.m[2]));
+}
+
 template <bool FAST, typename T, typename S>
 __device__ __forceinline__ void walk_init(const Ray<T> &r, const uint32_t n[3], const S st[3],
                                           Walk<FAST> &w, int scale = 1) {
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    w.r[k] = r.up[k] > 0 ? (int)n[k] - r.iv[k] : (r.up[k] < 0 ? r.iv[k] + 1 : kNoStep);
-    w.m[k] = -(r.up[k] * st[k] * scale);
-  }
-  if (FAST) {
+  if constexpr (FAST) {
+    int cnt[3], m[3];
     uint32_t lin = 0;
 #pragma unroll
-    for (int k = 0; k < 3; k++) lin += (uint32_t)r.iv[k] * (uint32_t)(st[k] * scale);
-    w.base = lin - ((uint32_t)(w.r[0] * w.m[0]) + (uint32_t)(w.r[1] * w.m[1]) +
-                    (uint32_t)(w.r[2] * w.m[2]));
+    for (int k = 0; k < 3; k++) {
+      cnt[k] = r.up[k] > 0 ? (int)n[k] - r.iv[k] : (r.up[k] < 0 ? r.iv[k] + 1 : kNoStep);
+      m[k] = -(r.up[k] * (int)st[k] * scale);
+      lin += (uint32_t)r.iv[k] * (uint32_t)(st[k] * scale);
+    }
+    walk_set(w, cnt, m, lin);
   } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      w.r[k] = r.up[k] > 0 ? (int)n[k] - r.iv[k] : (r.up[k] < 0 ? r.iv[k] + 1 : kNoStep);
+      w.m[k] = -(r.up[k] * st[k] * scale);
+    }
     int64_t lin = 0;
 #pragma unroll
-    for (int k = 0; k < 3; k++) lin += (int64_t)r.iv[k] * (int64_t)st[k];
+    for (int k = 0; k < 3; k++) lin += (int64_t)r.iv[k] * (int64_t)st[k] * scale;
     w.base = lin - ((int64_t)w.r[0] * w.m[0] + (int64_t)w.r[1] * w.m[1] + (int64_t)w.r[2] * w.m[2]);
   }
 }
@@ -307,14 +330,26 @@ __device__ __forceinline__ bool walk_inside(const Walk<FAST> &w) {
   return min(min(w.r[0], w.r[1]), w.r[2]) > 0;
 }
 
-// One step of C:811-912 on (D, ac) with the counters of up to two walks
-// (cube and voxel operands) instead of iv.  Same floating-point operations as
+// Offset in the array of walk `g` (its multipliers and base) at the counters of
+// walk `c`: a second array walked along the same ray (same shape, other
+// strides) has the same counters, so only one set is stepped.
+__device__ __forceinline__ uint32_t walk_off(const Walk<true> &g, const Walk<true> &c) {
+  return g.base + (uint32_t)__mul24(c.r[0], g.m[0]) + (uint32_t)__mul24(c.r[1], g.m[1]) +
+         (uint32_t)__mul24(c.r[2], g.m[2]);
+}
+
+__device__ __forceinline__ int64_t walk_off(const Walk<false> &g, const Walk<false> &c) {
+  return g.base + c.r[0] * g.m[0] + c.r[1] * g.m[1] + c.r[2] * g.m[2];
+}
+
+// One step of C:811-912 on (D, ac) with the walk's counters instead of iv
+// (a voxel-operand walk shares them, see walk_off).  Same floating-point operations as
 // step(): the minimum is taken with min3 (exact: no NaN reaches here).  The
 // reference's length is D of the first axis holding the minimum; the minimum
 // itself is that value, up to the sign of a zero (-0 == +0), and a zero
 // length adds a zero product that leaves any nonzero sum unchanged.
-template <typename T, bool AC, bool FAST, bool TWO>
-__device__ __forceinline__ T step_walk(Ray<T> &r, Walk<FAST> &w, Walk<FAST> &w2) {
+template <typename T, bool AC, bool FAST>
+__device__ __forceinline__ T step_walk(Ray<T> &r, Walk<FAST> &w) {
   const T d = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
   const bool sx = (r.D[0] == d);
   const bool sy = (r.D[1] == d);
@@ -326,11 +361,6 @@ __device__ __forceinline__ T step_walk(Ray<T> &r, Walk<FAST> &w, Walk<FAST> &w2)
   w.r[0] -= sx;
   w.r[1] -= sy;
   w.r[2] -= sz;
-  if (TWO) {
-    w2.r[0] -= sx;
-    w2.r[1] -= sy;
-    w2.r[2] -= sz;
-  }
   return d;
 }
 

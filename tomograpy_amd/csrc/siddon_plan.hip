@@ -67,11 +67,22 @@ __device__ __forceinline__ void ray_pixel(const BrickGeom &g, uint32_t n1, uint3
   }
 }
 
+// Per-ray constants of C:750-774 (|p| = |CDELT / u| and the step direction
+// of each axis), written by the plan build with the arithmetic of ray_setup
+// so that the backprojection need not recompute the unit vector and three
+// divisions for every record.  up = (up0 + 1) | (up1 + 1) << 2 | (up2 + 1) << 4.
+template <typename T>
+struct RayEntry {
+  T pabs[3];
+  int32_t up;
+};
+
 template <typename T>
 struct RecPtrs {
   uint32_t *rid;
   uint32_t *ivp;
   T *d0, *d1, *d2, *ac;
+  RayEntry<T> *ray;  // indexed by (t * n1 + i) * n2 + j
 };
 
 __device__ __forceinline__ int brick_of(const BrickGeom &g, const int iv[3], int bin) {
@@ -114,7 +125,7 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
     }
     const int b = brick_of(g, r.iv, bin);
     if (b != last) {
-      emit(b, r, rid);
+      emit(b, r, rid, last < 0);
       last = b;
     }
     bool sx, sy, sz;
@@ -132,7 +143,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
     __syncthreads();
   }
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &, uint32_t) {
+  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &, uint32_t, bool) {
     if (lds) atomicAdd(&hist[b], 1u);
     else atomicAdd(&counts[b], 1u);
   });
@@ -154,14 +165,24 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
   if (lds) {
     for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) slot[k] = 0;
     __syncthreads();
-    trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &, uint32_t) { atomicAdd(&slot[b], 1u); });
+    trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &, uint32_t, bool) { atomicAdd(&slot[b], 1u); });
     __syncthreads();
     // reserve one contiguous range per brick for this workgroup's records
     for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x)
       if (slot[k]) slot[k] = atomicAdd(&cursor[k], slot[k]);
     __syncthreads();
   }
-  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &r, uint32_t rid) {
+  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &r, uint32_t rid, bool first) {
+    if (first) {
+      int t, i, j;
+      ray_pixel(g, (uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
+      RayEntry<T> e;
+      e.pabs[0] = r.pabs[0];
+      e.pabs[1] = r.pabs[1];
+      e.pabs[2] = r.pabs[2];
+      e.up = (r.up[0] + 1) | ((r.up[1] + 1) << 2) | ((r.up[2] + 1) << 4);
+      rec.ray[((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j] = e;
+    }
     const uint32_t s = lds ? atomicAdd(&slot[b], 1u) : atomicAdd(&cursor[b], 1u);
     unsigned long long q = s;
     if (g.shuffle) {  // a bijection of [0, count): mult[b] is coprime with the count
@@ -217,19 +238,25 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
   for (unsigned long long p = r0 + threadIdx.x; p < r1; p += blockDim.x) {
     int t, i, j;
     ray_pixel(g, (uint32_t)a.n1, (uint32_t)a.n2, rec.rid[p], t, i, j);
-    const int64_t did = i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+    const int64_t did = a.data32 ? (int64_t)((uint32_t)i * (uint32_t)a.ds[0] +
+                                             (uint32_t)j * (uint32_t)a.ds[1] +
+                                             (uint32_t)t * (uint32_t)a.ds[2])
+                                 : i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
     T w = a.data[did];
     if (a.pixw) w = a.pixw[did] * w;
     if (w == (T)0) continue;
     const ViewHdr<T> &h = a.views[t];
     T u0[3];
-    pixel_unit_vector(a, h, t, i, j, u0);
+    if (SUN) pixel_unit_vector(a, h, t, i, j, u0);  // the obstacle test needs the direction
     Ray<T> r;
+    {
+      const RayEntry<T> e =
+          rec.ray[((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {  // the per-ray constants of C:750-774, as ray_setup computes them
-      const T pk = (u0[k] != 0) ? a.mp[k] / u0[k] : (T)kInf;
-      r.pabs[k] = fabs(pk);
-      r.up[k] = sgn(u0[k]);
+      for (int k = 0; k < 3; k++) {
+        r.pabs[k] = e.pabs[k];
+        r.up[k] = ((e.up >> (2 * k)) & 3) - 1;
+      }
     }
     const uint32_t v = rec.ivp[p];
     r.iv[0] = (int)(v & 1023u);
@@ -238,28 +265,31 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
     r.D[0] = rec.d0[p];
     r.D[1] = rec.d1[p];
     r.D[2] = rec.d2[p];
-    r.ac = rec.ac[p];
+    if (SUN) r.ac = rec.ac[p];
     // counters to the faces of brick & map (the record's voxel is inside both)
     Walk<true> wl;
     {
       const int lo[3] = {lo0, lo1, lo2}, ext[3] = {e0, e1, e2};
       // byte strides of the double accumulators
       const int ls[3] = {g.b[1] * g.b[2] * (int)sizeof(Acc), g.b[2] * (int)sizeof(Acc), (int)sizeof(Acc)};
+      int cnt[3], m[3];
       uint32_t l0 = 0;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        wl.r[k] = r.up[k] > 0 ? lo[k] + ext[k] - r.iv[k] : (r.up[k] < 0 ? r.iv[k] - lo[k] + 1 : kNoStep);
-        wl.m[k] = -(r.up[k] * ls[k]);
+        // both candidate counts, then selects (no branch per axis)
+        const int fwd = lo[k] + ext[k] - r.iv[k], back = r.iv[k] - lo[k] + 1;
+        int ck = r.up[k] > 0 ? fwd : back;
+        cnt[k] = r.up[k] == 0 ? kNoStep : ck;
+        m[k] = -(r.up[k] * ls[k]);
         l0 += (uint32_t)((r.iv[k] - lo[k]) * ls[k]);
       }
-      wl.base = l0 - ((uint32_t)(wl.r[0] * wl.m[0]) + (uint32_t)(wl.r[1] * wl.m[1]) +
-                      (uint32_t)(wl.r[2] * wl.m[2]));
+      walk_set(wl, cnt, m, l0);
     }
     int guard = g.b[0] + g.b[1] + g.b[2] + 2;
     while (walk_inside(wl) && (!SUN || outside_sun(h.M, u0, r.ac))) {
       if (--guard < 0) break;
       const uint32_t l = wl.off();
-      const T d = step_walk<T, SUN, true, false>(r, wl, wl);
+      const T d = step_walk<T, SUN, true>(r, wl);
       atomicAdd((Acc *)((unsigned char *)acc + l), (Acc)(d * w));
     }
   }
@@ -345,13 +375,18 @@ RecPtrs<T> rec_ptrs(const tg_bpj_plan *p) {
   r.d2 = (T *)(base + o);
   o = align_up(o + sizeof(T) * R, 256);
   r.ac = (T *)(base + o);
+  o = align_up(o + sizeof(T) * R, 256);
+  r.ray = (RayEntry<T> *)(base + o);
   return r;
 }
 
 template <typename T>
-size_t rec_bytes(size_t R) {
-  return 2 * align_up(4 * R, 256) + 4 * align_up(sizeof(T) * R, 256) + 256;
+size_t rec_bytes(size_t R, size_t rays) {
+  return 2 * align_up(4 * R, 256) + 4 * align_up(sizeof(T) * R, 256) +
+         align_up(sizeof(RayEntry<T>) * rays, 256) + 256;
 }
+
+size_t call_rays(const tg_call *c) { return (size_t)c->nv * c->n1 * c->n2; }
 
 template <typename T>
 int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
@@ -429,7 +464,7 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
   if ((rc = hip_check(hipStreamSynchronize(s), "plan meta sync"))) return rc;
   p->records = (int64_t)tot;
   p->chunks = (int64_t)ch;
-  p->rec_bytes = rec_bytes<T>(tot);
+  p->rec_bytes = rec_bytes<T>(tot, call_rays(c));
   for (int k = 0; k < 3; k++) p->nb_axis[k] = g.nb[k];
   p->nbricks = g.nbricks;
   return TG_OK;
@@ -444,8 +479,9 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   if ((rc = make_geom(c, p, g))) return rc;
   const MetaLayout m = meta_layout(g.nbricks);
   if (!p->meta || p->meta_bytes < m.total) return set_error(TG_EINVAL, "plan meta buffer too small");
-  if (!p->rec || p->rec_bytes < rec_bytes<T>((size_t)p->records))
-    return set_error(TG_EINVAL, "plan record buffer needs %zu bytes", rec_bytes<T>((size_t)p->records));
+  if (!p->rec || p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
+    return set_error(TG_EINVAL, "plan record buffer needs %zu bytes",
+                     rec_bytes<T>((size_t)p->records, call_rays(c)));
   char *meta = (char *)p->meta;
   uint32_t *cursor = (uint32_t *)(meta + m.cursor);
   const unsigned long long *offsets = (const unsigned long long *)(meta + m.offsets);
@@ -494,7 +530,7 @@ int plan_bpj(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   if ((rc = make_geom(c, p, g))) return rc;
   if (g.nbricks != p->nbricks) return set_error(TG_EINVAL, "plan was built for another geometry");
   if (!p->meta || p->meta_bytes < meta_layout(g.nbricks).total || !p->rec ||
-      p->rec_bytes < rec_bytes<T>((size_t)p->records))
+      p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
     return set_error(TG_EINVAL, "plan buffers missing or too small");
   const size_t lds = sizeof(Acc) * (size_t)g.b[0] * g.b[1] * g.b[2];
   if (lds > 160 * 1024) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);

@@ -74,11 +74,11 @@ __device__ __forceinline__ void project_tile(const KArgs<T> &a, uint32_t tile) {
         T cur = v0;
         bool keep = true;
         if (VOX) {
-          const auto lm = wm.off();
+          const auto lm = walk_off(wm, w);
           if (a.voxw) cur = a.voxw[lm] * cur;
           if (a.voxmask) keep = (a.voxmask[lm] == 0);
         }
-        const T d = step_walk<T, SUN, FAST, VOX>(r, w, wm);
+        const T d = step_walk<T, SUN, FAST>(r, w);
         v1 = buffer_load<T>(rs, w.off());
         if (keep) acc = acc + d * cur;
       }
@@ -87,11 +87,11 @@ __device__ __forceinline__ void project_tile(const KArgs<T> &a, uint32_t tile) {
         T cur = v1;
         bool keep = true;
         if (VOX) {
-          const auto lm = wm.off();
+          const auto lm = walk_off(wm, w);
           if (a.voxw) cur = a.voxw[lm] * cur;
           if (a.voxmask) keep = (a.voxmask[lm] == 0);
         }
-        const T d = step_walk<T, SUN, FAST, VOX>(r, w, wm);
+        const T d = step_walk<T, SUN, FAST>(r, w);
         v0 = buffer_load<T>(rs, w.off());
         if (keep) acc = acc + d * cur;
       }
@@ -115,11 +115,11 @@ __device__ __forceinline__ void project_tile(const KArgs<T> &a, uint32_t tile) {
       T v = cube[lin];
       bool keep = true;
       if (VOX) {
-        const auto lm = wm.off();
+        const auto lm = walk_off(wm, w);
         if (a.voxw) v = a.voxw[lm] * v;
         if (a.voxmask) keep = (a.voxmask[lm] == 0);
       }
-      const T d = step_walk<T, SUN, FAST, VOX>(r, w, wm);
+      const T d = step_walk<T, SUN, FAST>(r, w);
       if (keep) acc = acc + d * v;
     }
   }
@@ -152,9 +152,9 @@ __device__ __forceinline__ void scatter_ray(const KArgs<T> &a, const ViewHdr<T> 
     const auto lin = wk.off();
     bool keep = true;
     if (VOX) {
-      if (a.voxmask) keep = (a.voxmask[wm.off()] == 0);
+      if (a.voxmask) keep = (a.voxmask[walk_off(wm, wk)] == 0);
     }
-    const T d = step_walk<T, SUN, FAST, VOX>(r, wk, wm);
+    const T d = step_walk<T, SUN, FAST>(r, wk);
     if (keep) atomic_add(dst + lin, (A)(d * w));
   }
 }

@@ -98,6 +98,10 @@ typedef struct tg_call {
   size_t workspace_bytes;
   double *acc;              /* device, TG_ACC_F64: n0*n1*n2*nt doubles, zero on entry and
                                zero again when the call's work completes */
+  const uint32_t *ray_steps; /* device or NULL, pj/pjt: the Siddon step count of every ray,
+                                indexed (t * n1 + i) * n2 + j, as tg_count_steps writes it for
+                                this geometry (mask and obstacle included); lets the projector
+                                run each ray's loop for a known count */
 } tg_call;
 
 /* Size in bytes of the packed view descriptors for nv views of n1 x n2 pixels. */

@@ -44,7 +44,7 @@ class TgCall(ctypes.Structure):
                 ("tindex", _vp), ("pix_weight", _vp), ("vox_weight", _vp),
                 ("voxmask", _vp), ("vox_stride", _i64 * 3),
                 ("status", _vp), ("workspace", _vp), ("workspace_bytes", _sz),
-                ("acc", _vp)]
+                ("acc", _vp), ("ray_steps", _vp)]
 
 
 class TgCgSeg(ctypes.Structure):

@@ -208,6 +208,7 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a) {
   a.voxw = (const T *)c->vox_weight;
   a.voxmask = c->voxmask;
   a.status = c->status;
+  a.ray_steps = (c->kind == TG_PJ || c->kind == TG_PJT) ? c->ray_steps : nullptr;
   a.cube_bytes = buffer_walk_bytes(a.n, a.cs, (int)sizeof(T));
   {
     const int64_t dn[3] = {c->n1, c->n2, c->nv};

@@ -78,6 +78,7 @@ struct KArgs {
   int32_t *status;
   uint32_t *per_ray;
   unsigned long long *total;
+  const uint32_t *ray_steps;  // per-ray step counts (tg_call.ray_steps) or NULL
 };
 
 // Traversal state of one ray after the setup of C:561-573.

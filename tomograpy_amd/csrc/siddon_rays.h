@@ -51,7 +51,70 @@ __device__ __forceinline__ void project_tile(const KArgs<T> &a, uint32_t tile) {
   int64_t toff;
   if (!time_offset(a, t, toff)) return;
   T acc = (a.pixw || zero_out) ? (T)0 : a.data[did];
-  if (FAST && a.cube_bytes && in_map(a, r)) {
+  if (FAST && a.cube_bytes && a.ray_steps && in_map(a, r)) {
+    // The ray's step count is known (value independent, counted with the same
+    // traversal, obstacle included): the loop needs no per-axis counters, no
+    // exit test on them and no obstacle test; the voxel offsets move by the
+    // strides of the stepping axes.  Same steps and the same floating-point
+    // operations as the counter walk below.
+    const uint32_t nsteps =
+        a.ray_steps[((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j];
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(a.cube + toff), (short)0, (int)a.cube_bytes,
+                                          kBufferWord3);
+    int dm[3], dv[3];
+    uint32_t off = 0, offv = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      dm[k] = r.up[k] * (int)a.cs[k] * (int)sizeof(T);
+      off += (uint32_t)r.iv[k] * (uint32_t)a.cs[k] * (uint32_t)sizeof(T);
+      if (VOX) {
+        dv[k] = r.up[k] * (int)a.vs[k];
+        offv += (uint32_t)r.iv[k] * (uint32_t)a.vs[k];
+      }
+    }
+    T v0 = buffer_load<T>(rs, off), v1 = (T)0;
+    uint32_t s = 0;
+    while (true) {
+      if (s >= nsteps) break;
+      {
+        T cur = v0;
+        bool keep = true;
+        if (VOX) {
+          if (a.voxw) cur = a.voxw[offv] * cur;
+          if (a.voxmask) keep = (a.voxmask[offv] == 0);
+        }
+        const T d = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
+        const bool sx = (r.D[0] == d), sy = (r.D[1] == d), sz = (r.D[2] == d);
+        r.D[0] = sx ? r.pabs[0] : r.D[0] - d;
+        r.D[1] = sy ? r.pabs[1] : r.D[1] - d;
+        r.D[2] = sz ? r.pabs[2] : r.D[2] - d;
+        off += (uint32_t)((sx ? dm[0] : 0) + (sy ? dm[1] : 0) + (sz ? dm[2] : 0));
+        if (VOX) offv += (uint32_t)((sx ? dv[0] : 0) + (sy ? dv[1] : 0) + (sz ? dv[2] : 0));
+        v1 = buffer_load<T>(rs, off);
+        if (keep) acc = acc + d * cur;
+      }
+      if (++s >= nsteps) break;
+      {
+        T cur = v1;
+        bool keep = true;
+        if (VOX) {
+          if (a.voxw) cur = a.voxw[offv] * cur;
+          if (a.voxmask) keep = (a.voxmask[offv] == 0);
+        }
+        const T d = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
+        const bool sx = (r.D[0] == d), sy = (r.D[1] == d), sz = (r.D[2] == d);
+        r.D[0] = sx ? r.pabs[0] : r.D[0] - d;
+        r.D[1] = sy ? r.pabs[1] : r.D[1] - d;
+        r.D[2] = sz ? r.pabs[2] : r.D[2] - d;
+        off += (uint32_t)((sx ? dm[0] : 0) + (sy ? dm[1] : 0) + (sz ? dm[2] : 0));
+        if (VOX) offv += (uint32_t)((sx ? dv[0] : 0) + (sy ? dv[1] : 0) + (sz ? dv[2] : 0));
+        v0 = buffer_load<T>(rs, off);
+        if (keep) acc = acc + d * cur;
+      }
+      ++s;
+    }
+  } else if (FAST && a.cube_bytes && in_map(a, r)) {
     // Byte walk over a buffer resource, one voxel ahead: the load of the next
     // voxel is in flight while this step's cascade runs.  After the last step
     // the walk may name a voxel outside the cube; the resource's range check

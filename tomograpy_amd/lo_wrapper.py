@@ -72,7 +72,7 @@ class Siddon:
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
                  voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
                  tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0,
-                 plan_order=0, acc64=None):
+                 plan_order=0, acc64=None, counted_forward=True):
         self.data_header = list(data_header)
         self.map_header = dict(cube_header)
         check_headers(self.data_header, self.map_header)
@@ -128,6 +128,9 @@ class Siddon:
         # that the result does not depend on the order in which rays are added
         self.acc64 = (self.tdtype == torch.float32) if acc64 is None else bool(acc64)
         self._acc = None
+        # projections run each ray for its precounted number of steps
+        self.counted_forward = bool(counted_forward)
+        self._ray_steps = None
 
     # -- brick-privatised backprojection plan ------------------------------
     # In "auto" mode the plan is used when an atomic scatter would update each
@@ -201,7 +204,7 @@ class Siddon:
         return 0 if self._plan_bufs is None else sum(b.numel() for b in self._plan_bufs)
 
     # -- calls -------------------------------------------------------------
-    def _call(self, kind, cube, data):
+    def _call(self, kind, cube, data, counted=True):
         flags = self.flags
         if kind in (N.TG_BPJ, N.TG_BPJT) and self.deterministic:
             flags |= N.TG_DETERMINISTIC
@@ -218,7 +221,20 @@ class Siddon:
         elif kind in (N.TG_BPJ, N.TG_BPJT) and self.acc64 and self._dense(cube):
             c.flags |= N.TG_ACC_F64
             c.acc = self.acc_buffer().data_ptr()
+        if kind in (N.TG_PJ, N.TG_PJT) and counted and self.counted_forward:
+            c.ray_steps = self.ray_steps().data_ptr()
         return c
+
+    def ray_steps(self):
+        """Per-ray Siddon step counts of this geometry (device uint32, indexed
+        (t * n1 + i) * n2 + j), computed once and reused by every projection."""
+        if self._ray_steps is None:
+            c = self._call(N.TG_PJ, self.xin, self.xout, counted=False)
+            per = torch.zeros(self.nv * self.n1 * self.n2, dtype=torch.int32, device=self.dev)
+            N.check(N.lib().tg_count_steps(ctypes.byref(c), ctypes.c_void_p(per.data_ptr()), None,
+                                           S.stream_handle()))
+            self._ray_steps = per
+        return self._ray_steps
 
     def _dense(self, cube):
         """Whether `cube` has the dense layout TG_ACC_F64 needs."""
@@ -317,12 +333,8 @@ class Siddon:
     def count_steps(self):
         """Siddon steps of one application (value independent, cached)."""
         if getattr(self, "_steps", None) is None:
-            c = self._call(N.TG_PJ, self.xin, self.xout)
-            total = torch.zeros(1, dtype=torch.int64, device=self.dev)
-            per = torch.zeros((self.nv, self.n1, self.n2), dtype=torch.int32, device=self.dev)
-            N.check(N.lib().tg_count_steps(ctypes.byref(c), ctypes.c_void_p(per.data_ptr()),
-                                           ctypes.c_void_p(total.data_ptr()), S.stream_handle()))
-            self._steps = int(total.item())
+            per = self.ray_steps()
+            self._steps = int(per.to(torch.int64).sum().item())
             self._rays_traced = int((per > 0).sum().item())
         return self._steps
 
@@ -352,9 +364,9 @@ class Siddon4d(Siddon):
         self.shapein = self.cube_shape + (self.nt,)
         self.xin = map4_buffer(*self.cube_shape, self.nt, self.tdtype, self.dev)
 
-    def _call(self, kind, cube, data):
+    def _call(self, kind, cube, data, counted=True):
         kind = {N.TG_PJ: N.TG_PJT, N.TG_BPJ: N.TG_BPJT}.get(kind, kind)
-        c = super()._call(kind, cube, data)
+        c = super()._call(kind, cube, data, counted)
         c.map.nt = self.nt
         c.tindex = self.tindex.data_ptr()
         return c
@@ -372,14 +384,6 @@ class Siddon4d(Siddon):
         self._backproject_raw(y, out)
         return out
 
-    def count_steps(self):
-        if getattr(self, "_steps", None) is None:
-            c = self._call(N.TG_PJ, self.xin, self.xout)
-            total = torch.zeros(1, dtype=torch.int64, device=self.dev)
-            N.check(N.lib().tg_count_steps(ctypes.byref(c), None,
-                                           ctypes.c_void_p(total.data_ptr()), S.stream_handle()))
-            self._steps = int(total.item())
-        return self._steps
 
 
 def siddon_lo(data_header, cube_header, **kwargs):

@@ -3,13 +3,14 @@
 // A scatter of d * w into the cube with one global atomic per Siddon step is
 // bound by atomic throughput.  Here the cube is cut into bricks that fit in
 // LDS.  A one-time, value-independent plan traces every ray once and records,
-// at each brick the ray enters, the traversal state there (voxel index, the
-// three distances D and the parameter ac), grouped by brick.  A
+// at each brick the ray enters, the traversal state there (the voxel's index
+// inside the brick, the three distances D) and the number of steps the ray
+// takes in the brick (the Sun obstacle included), grouped by brick.  A
 // backprojection then runs one workgroup per chunk of a brick's records: each
-// lane restores a record, steps through the brick with exactly the
-// operations of the continuous traversal (so every length d is the one the
-// reference computes), adds d * w into the LDS brick, and the workgroup
-// flushes the brick with one coalesced atomic per touched voxel.
+// lane restores a record, takes its steps with exactly the operations of the
+// continuous traversal (so every length d is the one the reference computes),
+// adds d * w into the LDS brick, and the workgroup flushes the brick with one
+// coalesced atomic per touched voxel.
 //
 // Plan passes: (1) count records per brick (LDS histogram per workgroup,
 // flushed with one global atomic per touched brick); counts are copied to the
@@ -77,24 +78,33 @@ struct RayEntry {
   int32_t up;
 };
 
+// Records, structure of arrays: ray id, lq = (voxel index inside the brick) |
+// (steps in the brick) << 16, and the distances D at the brick entry.
 template <typename T>
 struct RecPtrs {
   uint32_t *rid;
-  uint32_t *ivp;
-  T *d0, *d1, *d2, *ac;
+  uint32_t *lq;
+  T *d0, *d1, *d2;
   RayEntry<T> *ray;  // indexed by (t * n1 + i) * n2 + j
 };
+
+__device__ __forceinline__ uint32_t brick_local(const BrickGeom &g, const int iv[3]) {
+  return ((uint32_t)(iv[0] & (g.b[0] - 1)) << (g.lg[1] + g.lg[2])) |
+         ((uint32_t)(iv[1] & (g.b[1] - 1)) << g.lg[2]) | (uint32_t)(iv[2] & (g.b[2] - 1));
+}
 
 __device__ __forceinline__ int brick_of(const BrickGeom &g, const int iv[3], int bin) {
   return ((((iv[0] >> g.lg[0]) * g.nb[1]) + (iv[1] >> g.lg[1])) * g.nb[2] + (iv[2] >> g.lg[2])) +
          bin * g.nb[0] * g.nb[1] * g.nb[2];
 }
 
-// Traces the ray of (tile, lane) and calls emit(brick, ray) at the first step
-// in every brick.  Same traversal as the projector kernels.
-template <typename T, bool SUN, class F>
+// Traces the ray of (tile, lane); calls emit(brick, ray, id, first) at the
+// first step in every brick and close(steps) when the ray leaves that brick
+// (or the map, or meets the Sun) with the number of steps taken in it.  Same
+// traversal as the projector kernels.
+template <typename T, bool SUN, class F, class G>
 __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom &g, uint32_t tile,
-                                             F &&emit) {
+                                             F &&emit, G &&close) {
   int i, j, t;
   if (!lane_pixel(a, tile, i, j, t)) return;
   if (a.pixmask && a.pixmask[i * a.ms[0] + j * a.ms[1] + (int64_t)t * a.ms[2]] != 0) return;
@@ -117,6 +127,7 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
   }
   const uint32_t rid = ray_id(g, (uint32_t)a.n1, (uint32_t)a.n2, t, i, j);
   int last = -1;
+  uint32_t seg = 0;
   int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
   while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
     if (--guard < 0) {
@@ -125,12 +136,16 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
     }
     const int b = brick_of(g, r.iv, bin);
     if (b != last) {
+      if (last >= 0) close(seg);
       emit(b, r, rid, last < 0);
       last = b;
+      seg = 0;
     }
     bool sx, sy, sz;
     step(r, sx, sy, sz);
+    seg++;
   }
+  if (last >= 0) close(seg);
 }
 
 template <typename T, bool SUN>
@@ -143,10 +158,13 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
     __syncthreads();
   }
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &, uint32_t, bool) {
-    if (lds) atomicAdd(&hist[b], 1u);
-    else atomicAdd(&counts[b], 1u);
-  });
+  trace_bricks<T, SUN>(
+      a, g, tile,
+      [&](int b, const Ray<T> &, uint32_t, bool) {
+        if (lds) atomicAdd(&hist[b], 1u);
+        else atomicAdd(&counts[b], 1u);
+      },
+      [](uint32_t) {});
   if (lds) {
     __syncthreads();
     for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x)
@@ -165,13 +183,17 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
   if (lds) {
     for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) slot[k] = 0;
     __syncthreads();
-    trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &, uint32_t, bool) { atomicAdd(&slot[b], 1u); });
+    trace_bricks<T, SUN>(
+        a, g, tile, [&](int b, const Ray<T> &, uint32_t, bool) { atomicAdd(&slot[b], 1u); },
+        [](uint32_t) {});
     __syncthreads();
     // reserve one contiguous range per brick for this workgroup's records
     for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x)
       if (slot[k]) slot[k] = atomicAdd(&cursor[k], slot[k]);
     __syncthreads();
   }
+  unsigned long long pcur = 0;
+  uint32_t lcur = 0;
   trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &r, uint32_t rid, bool first) {
     if (first) {
       int t, i, j;
@@ -191,12 +213,12 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
     }
     const unsigned long long p = offsets[b] + q;
     rec.rid[p] = rid;
-    rec.ivp[p] = (uint32_t)r.iv[0] | ((uint32_t)r.iv[1] << 10) | ((uint32_t)r.iv[2] << 20);
     rec.d0[p] = r.D[0];
     rec.d1[p] = r.D[1];
     rec.d2[p] = r.D[2];
-    rec.ac[p] = r.ac;
-  });
+    pcur = p;
+    lcur = brick_local(g, r.iv);
+  }, [&](uint32_t steps) { rec.lq[pcur] = lcur | (steps << 16); });
 }
 
 // The brick is accumulated in double for both element types: on gfx950 the
@@ -231,8 +253,6 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
   const int bs = b % nsp;
   const int bz = bs % g.nb[2], by = (bs / g.nb[2]) % g.nb[1], bx = bs / (g.nb[2] * g.nb[1]);
   const int lo0 = bx << g.lg[0], lo1 = by << g.lg[1], lo2 = bz << g.lg[2];
-  const int e0 = min(g.b[0], (int)a.n[0] - lo0), e1 = min(g.b[1], (int)a.n[1] - lo1),
-            e2 = min(g.b[2], (int)a.n[2] - lo2);
   for (int k = threadIdx.x; k < vol; k += blockDim.x) acc[k] = (Acc)0;
   __syncthreads();
   for (unsigned long long p = r0 + threadIdx.x; p < r1; p += blockDim.x) {
@@ -245,52 +265,26 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
     T w = a.data[did];
     if (a.pixw) w = a.pixw[did] * w;
     if (w == (T)0) continue;
-    const ViewHdr<T> &h = a.views[t];
-    T u0[3];
-    if (SUN) pixel_unit_vector(a, h, t, i, j, u0);  // the obstacle test needs the direction
-    Ray<T> r;
-    {
-      const RayEntry<T> e =
-          rec.ray[((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j];
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        r.pabs[k] = e.pabs[k];
-        r.up[k] = ((e.up >> (2 * k)) & 3) - 1;
-      }
-    }
-    const uint32_t v = rec.ivp[p];
-    r.iv[0] = (int)(v & 1023u);
-    r.iv[1] = (int)((v >> 10) & 1023u);
-    r.iv[2] = (int)((v >> 20) & 1023u);
-    r.D[0] = rec.d0[p];
-    r.D[1] = rec.d1[p];
-    r.D[2] = rec.d2[p];
-    if (SUN) r.ac = rec.ac[p];
-    // counters to the faces of brick & map (the record's voxel is inside both)
-    Walk<true> wl;
-    {
-      const int lo[3] = {lo0, lo1, lo2}, ext[3] = {e0, e1, e2};
-      // byte strides of the double accumulators
-      const int ls[3] = {g.b[1] * g.b[2] * (int)sizeof(Acc), g.b[2] * (int)sizeof(Acc), (int)sizeof(Acc)};
-      int cnt[3], m[3];
-      uint32_t l0 = 0;
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        // both candidate counts, then selects (no branch per axis)
-        const int fwd = lo[k] + ext[k] - r.iv[k], back = r.iv[k] - lo[k] + 1;
-        int ck = r.up[k] > 0 ? fwd : back;
-        cnt[k] = r.up[k] == 0 ? kNoStep : ck;
-        m[k] = -(r.up[k] * ls[k]);
-        l0 += (uint32_t)((r.iv[k] - lo[k]) * ls[k]);
-      }
-      walk_set(wl, cnt, m, l0);
-    }
-    int guard = g.b[0] + g.b[1] + g.b[2] + 2;
-    while (walk_inside(wl) && (!SUN || outside_sun(h.M, u0, r.ac))) {
-      if (--guard < 0) break;
-      const uint32_t l = wl.off();
-      const T d = step_walk<T, SUN, true>(r, wl);
-      atomicAdd((Acc *)((unsigned char *)acc + l), (Acc)(d * w));
+    // per-ray constants, the record's entry state and its step count
+    const RayEntry<T> e =
+        rec.ray[((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j];
+    T D0 = rec.d0[p], D1 = rec.d1[p], D2 = rec.d2[p];
+    const uint32_t lq = rec.lq[p];
+    // byte offsets in the double accumulators of one step along each axis
+    const int u0 = ((e.up >> 0) & 3) - 1, u1 = ((e.up >> 2) & 3) - 1, u2 = ((e.up >> 4) & 3) - 1;
+    const int dm0 = u0 * (int)(sizeof(Acc) << (g.lg[1] + g.lg[2]));
+    const int dm1 = u1 * (int)(sizeof(Acc) << g.lg[2]);
+    const int dm2 = u2 * (int)sizeof(Acc);
+    unsigned char *cell = (unsigned char *)(acc + (lq & 0xffffu));
+    const uint32_t n = lq >> 16;
+    for (uint32_t k = 0; k < n; k++) {
+      const T d = tmin(tmin(D0, D1), D2);
+      const bool sx = (D0 == d), sy = (D1 == d), sz = (D2 == d);
+      D0 = sx ? e.pabs[0] : D0 - d;
+      D1 = sy ? e.pabs[1] : D1 - d;
+      D2 = sz ? e.pabs[2] : D2 - d;
+      atomicAdd((Acc *)cell, (Acc)(d * w));
+      cell += (sx ? dm0 : 0) + (sy ? dm1 : 0) + (sz ? dm2 : 0);
     }
   }
   __syncthreads();
@@ -330,6 +324,9 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
     g.nb[k] = (c->map.n[k] + b - 1) / b;
     if (c->map.n[k] > 1024) return set_error(TG_EINVAL, "plan backprojection supports n <= 1024");
   }
+  // a record packs the voxel index inside the brick and the steps in it in 16 bits each
+  if ((long long)g.b[0] * g.b[1] * g.b[2] > 65536 || g.b[0] + g.b[1] + g.b[2] > 65535)
+    return set_error(TG_EINVAL, "brick of %d x %d x %d voxels is too large", g.b[0], g.b[1], g.b[2]);
   const long long nb = (long long)g.nb[0] * g.nb[1] * g.nb[2] * g.nt;
   if (nb > (1 << 30)) return set_error(TG_EINVAL, "too many bricks");
   g.nbricks = (int)nb;
@@ -366,7 +363,7 @@ RecPtrs<T> rec_ptrs(const tg_bpj_plan *p) {
   size_t o = 0;
   r.rid = (uint32_t *)(base + o);
   o = align_up(o + 4 * R, 256);
-  r.ivp = (uint32_t *)(base + o);
+  r.lq = (uint32_t *)(base + o);
   o = align_up(o + 4 * R, 256);
   r.d0 = (T *)(base + o);
   o = align_up(o + sizeof(T) * R, 256);
@@ -374,15 +371,13 @@ RecPtrs<T> rec_ptrs(const tg_bpj_plan *p) {
   o = align_up(o + sizeof(T) * R, 256);
   r.d2 = (T *)(base + o);
   o = align_up(o + sizeof(T) * R, 256);
-  r.ac = (T *)(base + o);
-  o = align_up(o + sizeof(T) * R, 256);
   r.ray = (RayEntry<T> *)(base + o);
   return r;
 }
 
 template <typename T>
 size_t rec_bytes(size_t R, size_t rays) {
-  return 2 * align_up(4 * R, 256) + 4 * align_up(sizeof(T) * R, 256) +
+  return 2 * align_up(4 * R, 256) + 3 * align_up(sizeof(T) * R, 256) +
          align_up(sizeof(RayEntry<T>) * rays, 256) + 256;
 }
 

@@ -182,7 +182,7 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a) {
   a.t1 = t1;
   a.ray_i = -1;
   a.ray_j = -1;
-  int tj = c->tile_j > 0 ? c->tile_j : 32;
+  int tj = c->tile_j > 0 ? c->tile_j : 16;
   if (tj > 256 || (tj & (tj - 1))) return set_error(TG_EINVAL, "tile_j must be a power of two <= 256");
   int lg = 0;
   while ((1 << lg) < tj) lg++;

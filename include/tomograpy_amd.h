@@ -165,12 +165,14 @@ int tg_scale_cube(const tg_call *c, void *stream);
 typedef struct tg_bpj_plan {
   int32_t brick[3];      /* brick edge per axis, powers of two; 0,0,0 = default (128 KiB) */
   int32_t chunk_records; /* records per workgroup chunk; 0 = chosen by tg_bpj_plan_count
-                            (about 1024 chunks of 2K..32K records) and written back */
+                            (about 1024 chunks of 2K..64K records) and written back */
   int32_t nb_axis[3];    /* bricks per axis (set by tg_bpj_plan_init) */
   int32_t nbricks;       /* bricks x time bins (set by tg_bpj_plan_init) */
   int32_t order;         /* 0 = records of a brick in trace order, 1 = permuted (spreads the
                             rays of a wavefront over the brick) */
-  int32_t pad;
+  int32_t trace_tile;    /* pixels per row of the ray tiles the plan build traces (power of
+                            two); sets which rays are neighbours in a brick's record list,
+                            hence in a wavefront of the backprojection; 0 = 256 */
   int64_t records;       /* set by tg_bpj_plan_count */
   int64_t chunks;        /* set by tg_bpj_plan_count */
   void *meta;            /* device, caller-owned, meta_bytes */

@@ -152,7 +152,7 @@ int hip_check(hipError_t e, const char *what) {
 
 // Validates *c and fills the kernel arguments for views [t0, t1).
 template <typename T>
-int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a) {
+int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a, int tile_j = 0) {
   memset(&a, 0, sizeof(a));
   if (!c) return set_error(TG_EINVAL, "null call");
   if (c->n1 < 0 || c->n2 < 0 || c->nv < 0)
@@ -182,7 +182,7 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a) {
   a.t1 = t1;
   a.ray_i = -1;
   a.ray_j = -1;
-  int tj = c->tile_j > 0 ? c->tile_j : 16;
+  int tj = tile_j > 0 ? tile_j : (c->tile_j > 0 ? c->tile_j : 16);
   if (tj > 256 || (tj & (tj - 1))) return set_error(TG_EINVAL, "tile_j must be a power of two <= 256");
   int lg = 0;
   while ((1 << lg) < tj) lg++;
@@ -233,8 +233,8 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a) {
   return TG_OK;
 }
 
-template int make_args<float>(const tg_call *, int, int, KArgs<float> &);
-template int make_args<double>(const tg_call *, int, int, KArgs<double> &);
+template int make_args<float>(const tg_call *, int, int, KArgs<float> &, int);
+template int make_args<double>(const tg_call *, int, int, KArgs<double> &, int);
 
 template <typename T>
 uint32_t grid_blocks(const KArgs<T> &a) {

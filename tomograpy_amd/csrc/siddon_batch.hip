@@ -23,7 +23,7 @@
 namespace tg {
 
 template <typename T>
-int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a);
+int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a, int tile_j = 0);
 
 namespace {
 

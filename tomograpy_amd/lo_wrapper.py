@@ -72,7 +72,7 @@ class Siddon:
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
                  voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
                  tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0,
-                 plan_order=0, acc64=None, counted_forward=True):
+                 plan_order=0, plan_trace_tile=0, acc64=None, counted_forward=True):
         self.data_header = list(data_header)
         self.map_header = dict(cube_header)
         check_headers(self.data_header, self.map_header)
@@ -120,6 +120,7 @@ class Siddon:
         self.brick = brick
         self.chunk_records = int(chunk_records)
         self.plan_order = int(plan_order)
+        self.plan_trace_tile = int(plan_trace_tile)
         self._plan = None
         self._plan_bufs = None
         self._plan_count = None
@@ -166,6 +167,7 @@ class Siddon:
                     p.brick[k] = int(self.brick[k])
             p.chunk_records = self.chunk_records
             p.order = self.plan_order
+            p.trace_tile = self.plan_trace_tile
             N.check(lib.tg_bpj_plan_init(ctypes.byref(c), ctypes.byref(p)))
             meta = torch.empty(max(int(p.meta_bytes), 1), dtype=torch.uint8, device=self.dev)
             p.meta = meta.data_ptr()

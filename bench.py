@@ -54,7 +54,79 @@ def parse():
     ap.add_argument("--serve-max-active", type=int, default=8)
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="Poisson arrival rate in requests/s (0 = all requests at once)")
+    ap.add_argument("--no-grad", action="store_true", help="skip the differentiation leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     return ap.parse_args()
+
+
+def _event_ms(fn, reps):
+    import torch
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def grad_leg():
+    """Differentiation workload: 128^3 x 8 time bins, 64 views of 256^2, voxel
+    mask r < 1.05; one evaluation of ||P x - y||^2 + 1e-2 ||grad x||^2 +
+    1e-2 ||dx/dt||^2 and its gradient (forward through the projector, backward
+    through the hand-written backprojector)."""
+    from tomograpy_amd import workloads as W
+    P, y, x0 = W.config3_problem(np.float32)
+    loss = W.config3_loss(P, y)
+    x = x0.clone().requires_grad_(True)
+
+    def step():
+        x.grad = None
+        loss(x).backward()
+
+    ms = _event_ms(step, 5)
+    return {"workload": "128^3 x 8 bins, 64 views of 256^2, voxel mask r<1.05, fp32",
+            "ms_per_loss_and_gradient": ms, "siddon_steps_per_projection": P.count_steps()}
+
+
+def batch_leg():
+    """Batched-jobs workload: 64 uneven projector + backprojector jobs (cubes
+    32^3-192^3, 8-96 views of 64^2-512^2, D 150-250, half Thomson-weighted),
+    in the persistent launch (autotuned tile and schedule) against the same
+    jobs one launch at a time."""
+    import torch
+    from tomograpy_amd import batch as B
+    from tomograpy_amd import workloads as W
+    ops, xs = W.config4_problem()
+    outs = [torch.zeros((P.nv, P.n1, P.n2), dtype=P.tdtype, device=P.dev).permute(1, 2, 0)
+            for P in ops]
+    cubes = [torch.zeros(tuple(P.shapein), dtype=P.tdtype, device=P.dev) for P in ops]
+    for P in ops:  # geometry set-up (step counts, plans) outside the timing
+        P.ray_steps()
+        if P.use_plan():
+            P.build_plan()
+    tuner = B.Autotuner(repeats=1)
+
+    def batched():
+        B.run_tuned(tuner, ops, xs, outs)
+        B.run(ops, outs, cubes, backward=True)
+
+    def one_by_one():
+        for P, x, o, c in zip(ops, xs, outs, cubes):
+            P.project_into(x, o)
+            P.backproject_into(o, c)
+
+    ms_b = _event_ms(batched, 3)
+    ms_1 = _event_ms(one_by_one, 3)
+    steps = sum(P.count_steps() for P in ops)
+    return {"workload": "64 jobs, cubes 32^3-192^3, 8-96 views of 64^2-512^2, half Thomson, fp32",
+            "jobs": len(ops), "ms_batched_pair": ms_b, "ms_one_by_one_pair": ms_1,
+            "speedup": ms_1 / ms_b, "siddon_steps_per_pair": 2 * steps,
+            "steps_per_s_batched": 2 * steps / (ms_b * 1e-3)}
 
 
 def serving_leg(args):
@@ -266,10 +338,17 @@ def main():
                        e0.elapsed_time(e1) / info["iterations"], "residual": info["residual"]}
 
     y_host = y.cpu().numpy() if not args.no_cpu else None
-    if args.serve_requests:
-        # release the pair's operator (its brick plan is 10.7 GB at 256^3) first
+    if args.serve_requests or not args.no_grad or not args.no_batch:
+        # release the pair's operator (its brick plan is 9.4 GB at 256^3) first
         del P, x, x_true, y, noise
         torch.cuda.empty_cache()
+    if not args.no_grad:
+        extra["differentiation"] = grad_leg()
+        torch.cuda.empty_cache()
+    if not args.no_batch:
+        extra["batch"] = batch_leg()
+        torch.cuda.empty_cache()
+    if args.serve_requests:
         extra["serving"] = serving_leg(args)
     cpu = None
     if not args.no_cpu:

@@ -192,6 +192,15 @@ int tg_bpj_plan_build(const tg_call *c, const tg_bpj_plan *p, void *stream);
 /* cube += P^T data through the plan (asynchronous). */
 int tg_bpj_plan_backproject(const tg_call *c, const tg_bpj_plan *p, void *stream);
 
+/* Bytes of the device queue and of the host staging buffer of a plan batch. */
+size_t tg_bpj_plan_batch_bytes(int32_t njobs);
+/* tg_bpj_plan_backproject for njobs (call, plan) pairs in one launch over all
+ * their chunks (one job table, copied from host_staging to queue on the stream;
+ * host_staging must stay untouched until that copy has run).  No reference
+ * counterpart: the batched-jobs pattern for the backprojector. */
+int tg_bpj_plan_batch(const tg_call *calls, const tg_bpj_plan *plans, int32_t njobs, void *queue,
+                      void *host_staging, void *stream);
+
 /* ---- device-resident conjugate gradient (lo.acg consumer, srt.py:225) ---- */
 
 /* out += lam * ((identity ? x : 0) + sum_k D_k^T D_k x) on a C-contiguous

@@ -91,6 +91,9 @@ SIGNATURES = [
     ("tg_bpj_plan_build", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan), _vp]),
     ("tg_bpj_plan_backproject", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan),
                                                _vp]),
+    ("tg_bpj_plan_batch_bytes", _sz, [_i32]),
+    ("tg_bpj_plan_batch", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan), _i32,
+                                         _vp, _vp, _vp]),
     ("tg_cg_seg_blocks", _i64, [_i64]),
     ("tg_cg_batch", ctypes.c_int, [_i32, _i32, _vp, _i32, _i64, _vp, _vp]),
     ("tg_batch_queue_bytes", _sz, [_i32]),

@@ -20,26 +20,59 @@ TILE_CHOICES = (8, 16, 32, 64)
 SCHEDULES = (0, 1)
 
 
-class _Buffers:
-    """Device queue + pinned host staging for a batch of up to njobs jobs."""
+class _Staging:
+    """A device job table and its pinned host staging buffer, with an event
+    recorded after the table's host-to-device copy is enqueued: the staging
+    buffer may be written again once that event has completed."""
 
-    def __init__(self, njobs, dev):
-        nbytes = int(N.lib().tg_batch_queue_bytes(njobs))
-        self.njobs = njobs
+    def __init__(self, nbytes, dev):
+        self.nbytes = nbytes
         self.queue = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         self.stage = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        self.copied = None
 
 
-_BUFFERS = {}
+class _Ring:
+    """A few staging buffers used in turn, so that a launch rarely waits for the
+    copy of the previous one (a stream synchronisation per launch otherwise)."""
+
+    DEPTH = 4
+
+    def __init__(self):
+        self.slots = []
+        self.i = 0
+
+    def take(self, nbytes, dev):
+        if len(self.slots) < self.DEPTH:
+            self.slots.append(None)
+        k = self.i % len(self.slots)
+        self.i += 1
+        b = self.slots[k]
+        if b is None or b.nbytes < nbytes:
+            if b is not None and b.copied is not None:
+                b.copied.synchronize()
+            b = _Staging(max(nbytes, 1 << 16), dev)
+            self.slots[k] = b
+        elif b.copied is not None:
+            b.copied.synchronize()
+        return b
+
+    def release(self, b):
+        """Marks the launch that read b.stage as enqueued on the current stream."""
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream())
+        b.copied = ev
 
 
-def _buffers(njobs, dev):
-    key = (str(dev),)
-    b = _BUFFERS.get(key)
-    if b is None or b.njobs < njobs:
-        b = _Buffers(max(njobs, 16), dev)
-        _BUFFERS[key] = b
-    return b
+_RINGS = {}
+
+
+def _ring(kind, dev):
+    key = (kind, str(dev))
+    r = _RINGS.get(key)
+    if r is None:
+        r = _RINGS[key] = _Ring()
+    return r
 
 
 def make_plan(tile_j=0, blocks=0, schedule=1):
@@ -50,8 +83,11 @@ def make_plan(tile_j=0, blocks=0, schedule=1):
     return p
 
 
-def run(ops, ins, outs, backward=False, plan=None, stream=None, zero_out=False):
-    """outs[k] += P_k ins[k] (or P_k^T ins[k] when backward) for all k, one launch.
+def run(ops, ins, outs, backward=False, plan=None, zero_out=False):
+    """outs[k] += P_k ins[k] (or P_k^T ins[k] when backward) for all k, on the
+    current stream: one persistent launch for a forward batch; for a backward
+    batch, one launch over the brick plans of the jobs that use one and one
+    persistent atomic launch over the others.
 
     ops: Siddon operators (same dtype); ins/outs: device tensors shaped like the
     operator's cube / image stack (outputs are accumulated into, as in the
@@ -65,6 +101,7 @@ def run(ops, ins, outs, backward=False, plan=None, stream=None, zero_out=False):
     calls = (N.TgCall * n)()
     post = []
     copies = []
+    alive = []  # temporaries the launches read: referenced until they are enqueued
     for k, (P, x, y) in enumerate(zip(ops, ins, outs)):
         if backward:
             target = y
@@ -72,6 +109,7 @@ def run(ops, ins, outs, backward=False, plan=None, stream=None, zero_out=False):
                 target = torch.zeros_like(y)
                 post.append((P, target, y))
             data = P._as_stack(x)
+            alive.append(data)
             c = P._call(N.TG_BPJ, target, data)
         else:
             target = y
@@ -79,17 +117,40 @@ def run(ops, ins, outs, backward=False, plan=None, stream=None, zero_out=False):
                 # pixel weights live in the operator's view-major layout
                 target = P._as_stack(y)
                 copies.append((target, y))
-            c = P._call(N.TG_PJ, P._as_cube(x), target)
+            cube = P._as_cube(x)
+            alive.append(cube)
+            c = P._call(N.TG_PJ, cube, target)
             if zero_out:
                 c.flags |= N.TG_ZERO_OUT
         ctypes.memmove(ctypes.byref(calls[k]), ctypes.byref(c), ctypes.sizeof(N.TgCall))
-    bufs = _buffers(n, ops[0].dev)
-    plan = plan if plan is not None else make_plan()
-    s = S.stream_handle() if stream is None else stream
-    N.check(N.lib().tg_batch_run(calls, n, ctypes.byref(plan), bufs.queue.data_ptr(),
-                                 bufs.stage.data_ptr(), s))
-    # the staging buffer is reused by the next launch: wait until the copy left it
-    torch.cuda.current_stream().synchronize()
+    s = S.stream_handle()
+    dev = ops[0].dev
+    if backward:
+        # brick-plan jobs in one plan launch, the others in one atomic launch
+        planned = [k for k, P in enumerate(ops) if P.use_plan()]
+        rest = sorted(set(range(n)) - set(planned))
+        if planned:
+            m = len(planned)
+            pcalls = (N.TgCall * m)()
+            plans = (N.TgBpjPlan * m)()
+            for q, k in enumerate(planned):
+                ctypes.memmove(ctypes.byref(pcalls[q]), ctypes.byref(calls[k]), ctypes.sizeof(N.TgCall))
+                p = ops[k].build_plan()
+                ctypes.memmove(ctypes.byref(plans[q]), ctypes.byref(p), ctypes.sizeof(N.TgBpjPlan))
+            ring = _ring("plan", dev)
+            b = ring.take(int(N.lib().tg_bpj_plan_batch_bytes(m)), dev)
+            N.check(N.lib().tg_bpj_plan_batch(pcalls, plans, m, b.queue.data_ptr(),
+                                              b.stage.data_ptr(), s))
+            ring.release(b)
+        if rest:
+            m = len(rest)
+            acalls = (N.TgCall * m)()
+            for q, k in enumerate(rest):
+                ctypes.memmove(ctypes.byref(acalls[q]), ctypes.byref(calls[k]), ctypes.sizeof(N.TgCall))
+            _batch_launch(acalls, m, plan, s, dev)
+    else:
+        _batch_launch(calls, n, plan, s, dev)
+    del alive
     for tmp, y in copies:
         y.copy_(tmp)
     for P, tmp, y in post:
@@ -99,6 +160,15 @@ def run(ops, ins, outs, backward=False, plan=None, stream=None, zero_out=False):
         N.check(N.lib().tg_scale_cube(ctypes.byref(sc), S.stream_handle()))
         y.add_(tmp)
     return outs
+
+
+def _batch_launch(calls, n, plan, s, dev):
+    ring = _ring("batch", dev)
+    b = ring.take(int(N.lib().tg_batch_queue_bytes(n)), dev)
+    plan = plan if plan is not None else make_plan()
+    N.check(N.lib().tg_batch_run(calls, n, ctypes.byref(plan), b.queue.data_ptr(),
+                                 b.stage.data_ptr(), s))
+    ring.release(b)
 
 
 def signature(ops, backward):

@@ -10,9 +10,10 @@ unpinned; this is textbook CG (Hestenes-Stiefel) stopped on ||r|| <= tol ||r0||.
 Every vector and scalar stays on the GPU.  One solve and many concurrent
 solves (the reconstruction server) run the same kernels: the segmented CG
 kernels (``tg_cg_batch``) take a table of solves, and the projector pair runs
-as one-job launches for one solve or one batched launch for many.  A solve
-therefore computes the same iterates alone or in a mix, up to the summation
-order of the atomic backprojection.
+as one-job launches for one solve or batched launches for many.  A solve
+computes the same iterates alone or in a mix: the forward projection is
+bit-identical either way, and float32 backprojections sum in float64
+(``TG_ACC_F64``), which makes them independent of the summation order.
 """
 import ctypes
 
@@ -147,22 +148,11 @@ def iterate(solves, table, tuner=None):
         outs = [s.out for s in solves]
         plan_f = tuner.plan_for(ops, ins, outs, False, zero_out=True) if tuner else None
         B.run(ops, ins, outs, backward=False, plan=plan_f, zero_out=True)
-        # large cubes backproject through their brick plans (an LDS-privatised
-        # launch each); the rest share one batched atomic launch
-        rest = []
-        for s in solves:
-            if s.P.use_plan():
-                _backward_one(s)
-            else:
-                rest.append(s)
-        if len(rest) == 1:
-            _backward_one(rest[0])
-        elif rest:
-            ops = [s.P for s in rest]
-            outs = [s.out for s in rest]
-            qs = [s.view(s.qs) for s in rest]
-            plan_b = tuner.plan_for(ops, outs, qs, True) if tuner else None
-            B.run(ops, outs, qs, backward=True, plan=plan_b)
+        # contended geometries backproject through their brick plans in one
+        # launch, the others in one persistent atomic launch
+        qs = [s.view(s.qs) for s in solves]
+        plan_b = tuner.plan_for(ops, outs, qs, True) if tuner else None
+        B.run(ops, outs, qs, backward=True, plan=plan_b)
     table.op(N.TG_CG_OP_REGULARIZE)
     table.op(N.TG_CG_OP_DOT_PQ)
     table.op(N.TG_CG_OP_UPDATE)

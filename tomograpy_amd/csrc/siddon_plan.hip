@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <string>
 #include <vector>
 
 #include "siddon_common.h"
@@ -35,6 +36,7 @@ namespace {
 constexpr int kPlanBlock = 256;
 constexpr int kHistMax = 8192;  // bricks x bins histogrammed in LDS; above: global atomics
 constexpr int kBpjBlock = 1024;
+constexpr size_t kPlanQueueHeader = 256;
 // Default pixels per row of the ray tiles traced by the plan build: one image
 // row of 256 pixels per workgroup, so that the 64 records a wavefront of the
 // backprojection walks together come from 64 adjacent pixels of a row --
@@ -233,17 +235,16 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
 // formed in T exactly as the reference forms it.
 using Acc = double;
 
-// One workgroup per chunk of one brick's records.
-template <typename T, bool SUN>
-__global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
-                                                        const unsigned long long *offsets,
-                                                        const unsigned long long *chunk_prefix,
-                                                        RecPtrs<T> rec) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  Acc *acc = reinterpret_cast<Acc *>(smem_raw);
+// The work of one workgroup: chunk c of one brick's records, accumulated in
+// the LDS array acc and flushed to the cube (or to the float64 accumulator).
+template <typename T>
+__device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeom &g,
+                                               const unsigned long long *offsets,
+                                               const unsigned long long *chunk_prefix,
+                                               const RecPtrs<T> &rec, unsigned long long c,
+                                               Acc *acc) {
   const int vol = g.b[0] * g.b[1] * g.b[2];
   // chunk -> brick (binary search over the chunk prefix, uniform per workgroup)
-  const unsigned long long c = blockIdx.x;
   int lo = 0, hi = g.nbricks - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -306,6 +307,43 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
     if (a.acc) unsafeAtomicAdd(a.acc + off, (double)sum);
     else unsafeAtomicAdd(a.cube + off, (T)sum);
   }
+}
+
+// One workgroup per chunk of one brick's records.
+template <typename T>
+__global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
+                                                        const unsigned long long *offsets,
+                                                        const unsigned long long *chunk_prefix,
+                                                        RecPtrs<T> rec) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  plan_bpj_chunk<T>(a, g, offsets, chunk_prefix, rec, blockIdx.x, reinterpret_cast<Acc *>(smem_raw));
+}
+
+// Many plans in one launch: the chunks of all jobs, one after the other.
+template <typename T>
+struct PlanJob {
+  KArgs<T> a;
+  BrickGeom g;
+  const unsigned long long *offsets;
+  const unsigned long long *chunk_prefix;
+  RecPtrs<T> rec;
+  unsigned long long chunk_begin;  // first global chunk of this job
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_batch(const PlanJob<T> *__restrict__ jobs,
+                                                              int njobs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const unsigned long long c = blockIdx.x;
+  int lo = 0, hi = njobs - 1;  // job of this chunk (uniform per workgroup)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].chunk_begin <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  const PlanJob<T> &j = jobs[lo];
+  plan_bpj_chunk<T>(j.a, j.g, j.offsets, j.chunk_prefix, j.rec, c - j.chunk_begin,
+                    reinterpret_cast<Acc *>(smem_raw));
 }
 
 __global__ void k_zero_u32(uint32_t *p, int n) {
@@ -501,19 +539,26 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   return hip_check(hipGetLastError(), "plan fill launch");
 }
 
-template <typename T, bool SUN>
+// Lets a kernel use up to 160 KiB of dynamic LDS (once per kernel).
+template <class K>
+int allow_full_lds(K kernel, bool &done) {
+  if (done) return TG_OK;
+  hipError_t e = hipFuncSetAttribute((const void *)kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return hip_check(e, "plan bpj LDS attribute");
+  done = true;
+  return TG_OK;
+}
+
+template <typename T>
 int launch_plan_bpj(const KArgs<T> &a, const BrickGeom &g, const tg_bpj_plan *p, hipStream_t s) {
   const MetaLayout m = meta_layout(g.nbricks);
   const char *meta = (const char *)p->meta;
   const size_t lds = sizeof(Acc) * (size_t)g.b[0] * g.b[1] * g.b[2];
   static bool attr_set = false;
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_plan_bpj<T, SUN>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return hip_check(e, "plan bpj LDS attribute");
-    attr_set = true;
-  }
-  hipLaunchKernelGGL((k_plan_bpj<T, SUN>), dim3((uint32_t)p->chunks), dim3(kBpjBlock), lds, s, a, g,
+  int rc = allow_full_lds(k_plan_bpj<T>, attr_set);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_plan_bpj<T>), dim3((uint32_t)p->chunks), dim3(kBpjBlock), lds, s, a, g,
                      (const unsigned long long *)(meta + m.offsets),
                      (const unsigned long long *)(meta + m.chunk_prefix), rec_ptrs<T>(p));
   return hip_check(hipGetLastError(), "plan bpj launch");
@@ -536,10 +581,79 @@ int plan_bpj(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   const size_t lds = sizeof(Acc) * (size_t)g.b[0] * g.b[1] * g.b[2];
   if (lds > 160 * 1024) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
   if (p->chunks == 0) return TG_OK;
-  rc = (c->flags & TG_OBSTACLE_SUN) ? launch_plan_bpj<T, true>(a, g, p, s)
-                                    : launch_plan_bpj<T, false>(a, g, p, s);
-  if (rc) return rc;
+  // the Sun's effect is in the records' step counts: one kernel for both obstacles
+  if ((rc = launch_plan_bpj<T>(a, g, p, s))) return rc;
   return finish_acc<T>(a, s);
+}
+
+size_t plan_job_bytes() {
+  return align_up(sizeof(PlanJob<double>) > sizeof(PlanJob<float>) ? sizeof(PlanJob<double>)
+                                                                    : sizeof(PlanJob<float>),
+                  16);
+}
+
+// Checks one job of a plan batch and fills its table entry (chunk_begin aside).
+template <typename T>
+int plan_job(const tg_call *c, const tg_bpj_plan *p, PlanJob<T> &j, size_t &lds) {
+  if (c->kind != TG_BPJ && c->kind != TG_BPJT)
+    return set_error(TG_EINVAL, "plan backprojection needs kind bpj or bpjt");
+  int rc = make_args<T>(c, 0, c->nv, j.a);
+  if (rc) return rc;
+  if (!c->cube || !c->data) return set_error(TG_EINVAL, "cube and data pointers are required");
+  if ((rc = make_geom(c, p, j.g))) return rc;
+  if (j.g.nbricks != p->nbricks) return set_error(TG_EINVAL, "plan was built for another geometry");
+  if (!p->meta || p->meta_bytes < meta_layout(j.g.nbricks).total || !p->rec ||
+      p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
+    return set_error(TG_EINVAL, "plan buffers missing or too small");
+  lds = sizeof(Acc) * (size_t)j.g.b[0] * j.g.b[1] * j.g.b[2];
+  if (lds > 160 * 1024) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
+  const MetaLayout m = meta_layout(j.g.nbricks);
+  const char *meta = (const char *)p->meta;
+  j.offsets = (const unsigned long long *)(meta + m.offsets);
+  j.chunk_prefix = (const unsigned long long *)(meta + m.chunk_prefix);
+  j.rec = rec_ptrs<T>(p);
+  return TG_OK;
+}
+
+template <typename T>
+int plan_bpj_batch(const tg_call *calls, const tg_bpj_plan *plans, int njobs, void *queue,
+                   void *staging, hipStream_t s) {
+  memset(staging, 0, kPlanQueueHeader);
+  PlanJob<T> *table = (PlanJob<T> *)((char *)staging + kPlanQueueHeader);
+  unsigned long long total = 0;
+  size_t lds = 0;
+  for (int k = 0; k < njobs; k++) {
+    PlanJob<T> j;
+    memset(&j, 0, sizeof(j));
+    size_t l = 0;
+    int rc = plan_job<T>(&calls[k], &plans[k], j, l);
+    if (rc) {
+      const std::string msg = tg_last_error();
+      return set_error(rc, "job %d: %s", k, msg.c_str());
+    }
+    if (calls[k].dtype != calls[0].dtype) return set_error(TG_EINVAL, "batch jobs must share a dtype");
+    j.chunk_begin = total;
+    total += (unsigned long long)plans[k].chunks;
+    lds = l > lds ? l : lds;
+    memcpy(&table[k], &j, sizeof(j));
+  }
+  if (!queue) return TG_OK;
+  const size_t bytes = kPlanQueueHeader + (size_t)njobs * sizeof(PlanJob<T>);
+  int rc;
+  if ((rc = hip_check(hipMemcpyAsync(queue, staging, bytes, hipMemcpyHostToDevice, s),
+                      "plan batch table copy")))
+    return rc;
+  if (total > 0) {
+    if (total > 0x7fffffffull) return set_error(TG_EINVAL, "plan batch has too many chunks");
+    static bool attr_set = false;
+    if ((rc = allow_full_lds(k_plan_bpj_batch<T>, attr_set))) return rc;
+    hipLaunchKernelGGL((k_plan_bpj_batch<T>), dim3((uint32_t)total), dim3(kBpjBlock), lds, s,
+                       (const PlanJob<T> *)((const char *)queue + kPlanQueueHeader), njobs);
+    if ((rc = hip_check(hipGetLastError(), "plan batch launch"))) return rc;
+  }
+  for (int k = 0; k < njobs; k++)
+    if ((rc = finish_acc<T>(table[k].a, s))) return rc;
+  return TG_OK;
 }
 
 }  // namespace
@@ -591,6 +705,21 @@ int tg_bpj_plan_backproject(const tg_call *c, const tg_bpj_plan *p, void *stream
   if (c->dtype == TG_F32) return plan_bpj<float>(c, p, s);
   if (c->dtype == TG_F64) return plan_bpj<double>(c, p, s);
   return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+size_t tg_bpj_plan_batch_bytes(int32_t njobs) {
+  if (njobs < 0) return 0;
+  return kPlanQueueHeader + (size_t)njobs * plan_job_bytes();
+}
+
+int tg_bpj_plan_batch(const tg_call *calls, const tg_bpj_plan *plans, int32_t njobs, void *queue,
+                      void *host_staging, void *stream) {
+  if (njobs <= 0 || !calls || !plans) return set_error(TG_EINVAL, "empty plan batch");
+  if (!queue || !host_staging) return set_error(TG_EINVAL, "plan batch needs queue and staging buffers");
+  hipStream_t s = (hipStream_t)stream;
+  if (calls[0].dtype == TG_F32) return plan_bpj_batch<float>(calls, plans, njobs, queue, host_staging, s);
+  if (calls[0].dtype == TG_F64) return plan_bpj_batch<double>(calls, plans, njobs, queue, host_staging, s);
+  return set_error(TG_EINVAL, "bad dtype %d", calls[0].dtype);
 }
 
 }  // extern "C"

@@ -167,3 +167,54 @@ def config4_jobs(n_jobs=64, dtype=np.float32, max_n=192, max_views=96, pix=(64, 
         jobs.append({"cube_header": ch, "headers": hs, "thomson": bool(rng.integers(0, 2)),
                      "n": n, "nv": nv, "npix": npx})
     return jobs
+
+
+def config3_problem(dtype=np.float32, **sizes):
+    """Differentiation workload: (operator, data y, start x0) on the C3 geometry
+    (128^3 x 8 time bins, 64 views of 256^2 round-robin to the bins, voxels with
+    r < 1.05 excluded).  y = P x_true + 1% noise with a time-scaled blob corona."""
+    import torch
+    from .lo_wrapper import Siddon4d
+    ch, hs, tindex, voxmask, rng = config3_geometry(dtype, **sizes)
+    nt = int(tindex.max()) + 1
+    P = Siddon4d(hs, ch, tindex=tindex, voxmask=voxmask, dtype=dtype)
+    base = blob_phantom(ch, 16, rng, dtype=np.float64)
+    x_true = np.stack([base * (1 + 0.05 * b) for b in range(nt)], axis=-1).astype(dtype)
+    y = P.matvec(torch.from_numpy(x_true).to(P.dev)).clone()
+    y += 0.01 * float(y.abs().max()) * torch.from_numpy(
+        rng.standard_normal(tuple(y.shape)).astype(dtype)).to(P.dev)
+    x0 = torch.from_numpy(rng.uniform(0, 1, x_true.shape).astype(dtype)).to(P.dev)
+    return P, y, x0
+
+
+def config3_loss(P, y, lam=1e-2, lam_t=1e-2):
+    """x -> ||P x - y||^2 + lam ||grad_xyz x||^2 + lam_t ||d x / dt||^2 (differentiable)."""
+    import torch
+    from .autograd import project
+
+    def loss(x):
+        r = project(x, P) - y
+        out = (r * r).sum()
+        for ax in range(3):
+            out = out + lam * (torch.diff(x, dim=ax) ** 2).sum()
+        return out + lam_t * (torch.diff(x, dim=3) ** 2).sum()
+
+    return loss
+
+
+def config4_problem(dtype=np.float32, **kw):
+    """Batched-jobs workload: operators and phantoms of the C4 job list (half of
+    the jobs Thomson-weighted)."""
+    import torch
+    from . import models as Mo
+    from .lo_wrapper import Siddon
+    rng = rng_for(40)
+    ops, xs = [], []
+    for job in config4_jobs(dtype=dtype, **kw):
+        if job["thomson"]:
+            P = Mo.thomson_lo(job["headers"], job["cube_header"], dtype=dtype)
+        else:
+            P = Siddon(job["headers"], job["cube_header"], dtype=dtype)
+        ops.append(P)
+        xs.append(torch.from_numpy(blob_phantom(job["cube_header"], 8, rng, dtype=dtype)).to(P.dev))
+    return ops, xs

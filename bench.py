@@ -19,6 +19,9 @@ GPU pair with the oracle's output on that sample.
 serving: the mixed-size reconstruction stream (see serving_leg), served by
 the continuous-batching server and solved one request at a time: p50/p99
 latency, throughput, and the largest relative L2 between the two results.
+cg: 50 CG iterations of the regularised normal equations on the workload, in
+fp32 and in fp64 (with the fp64 pair time).  differentiation, batch: the
+autograd workload (grad_leg) and the 64 batched jobs (batch_leg).
 """
 import argparse
 import json
@@ -46,7 +49,8 @@ def parse():
     ap.add_argument("--tile-j", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cg-iters", type=int, default=0, help="also time this many CG iterations")
+    ap.add_argument("--cg-iters", type=int, default=50,
+                    help="CG iterations timed on the workload in fp32 and fp64 (0 = skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=32,
@@ -72,6 +76,45 @@ def _event_ms(fn, reps):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def cg_run(P, y, iters):
+    """Times `iters` CG iterations on (P^T P + 1e-2 (I + sum D_k^T D_k)) x = P^T y."""
+    import torch
+    from tomograpy_amd import cg as CG
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    _, info = CG.solve(P, y, lam=1e-2, identity=True, maxiter=iters)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return {"iterations": info["iterations"], "ms_per_iteration": e0.elapsed_time(e1) / iters,
+            "relative_residual": info["residual"]}
+
+
+def fp64_leg(args):
+    """The same workload in float64: the pair and the CG iterations."""
+    import torch
+    from tomograpy_amd import workloads as W
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    ch, hs, rng = W.config2_geometry(np.float64, n=args.n, n_views=args.views, n_pix=args.pix)
+    phantom = W.blob_phantom(ch, 40, rng, dtype=np.float64)
+    P = siddon_lo(hs, ch, dtype=np.float64)
+    x = torch.from_numpy(phantom).cuda()
+    y = P.matvec(x).clone()
+    y += 0.01 * float(y.max()) * torch.from_numpy(rng.normal(size=tuple(y.shape))).cuda()
+    if P.use_plan():
+        P.build_plan()
+
+    def pair():
+        P.project_into(x, P.xout)
+        P.backproject_into(y, P.xin)
+
+    out = {"ms_pair": _event_ms(pair, 5), "siddon_steps_per_projection": P.count_steps()}
+    out.update(cg_run(P, y, args.cg_iters))
+    return out
 
 
 def grad_leg():
@@ -240,7 +283,6 @@ def main():
     import tomograpy_amd as T
     from tomograpy_amd import workloads as W
     from tomograpy_amd.lo_wrapper import siddon_lo
-    from tomograpy_amd import cg as CG
 
     torch.cuda.set_device(0)
     dtype = np.dtype(args.dtype)
@@ -327,20 +369,15 @@ def main():
 
     extra = {}
     if args.cg_iters:
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        _, info = CG.solve(P, y, lam=1e-2, identity=True, maxiter=args.cg_iters)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        extra["cg"] = {"iterations": info["iterations"], "ms_per_iteration":
-                       e0.elapsed_time(e1) / info["iterations"], "residual": info["residual"]}
+        extra["cg"] = {"fp32": cg_run(P, y, args.cg_iters)}
 
     y_host = y.cpu().numpy() if not args.no_cpu else None
-    if args.serve_requests or not args.no_grad or not args.no_batch:
+    if args.serve_requests or not args.no_grad or not args.no_batch or args.cg_iters:
         # release the pair's operator (its brick plan is 9.4 GB at 256^3) first
         del P, x, x_true, y, noise
+        torch.cuda.empty_cache()
+    if args.cg_iters and dtype == np.float32:
+        extra["cg"]["fp64"] = fp64_leg(args)
         torch.cuda.empty_cache()
     if not args.no_grad:
         extra["differentiation"] = grad_leg()

@@ -169,7 +169,9 @@ typedef struct tg_bpj_plan {
   int32_t nb_axis[3];    /* bricks per axis (set by tg_bpj_plan_init) */
   int32_t nbricks;       /* bricks x time bins (set by tg_bpj_plan_init) */
   int32_t order;         /* 0 = records of a brick in trace order, 1 = permuted (spreads the
-                            rays of a wavefront over the brick) */
+                            rays of a wavefront over the brick), 2 = trace order with each
+                            pass of the backprojection workgroup (1024 records) sorted by
+                            decreasing step count (even walks within a wavefront) */
   int32_t trace_tile;    /* pixels per row of the ray tiles the plan build traces (power of
                             two); sets which rays are neighbours in a brick's record list,
                             hence in a wavefront of the backprojection; 0 = 256 */

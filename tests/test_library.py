@@ -186,7 +186,7 @@ def test_bpj_plan_init_fields(dtype_code, n, brick):
     N.check(N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)))
     b = list(p.brick)
     if brick == (0, 0, 0):
-        assert b == [16, 32, 32]  # 128 KiB of double accumulators
+        assert b == [32, 16, 32]  # 128 KiB of double accumulators
     nb = [(n + b[k] - 1) // b[k] for k in range(3)]
     assert list(p.nb_axis) == nb
     assert p.nbricks == nb[0] * nb[1] * nb[2] * 3

@@ -53,6 +53,7 @@ struct BrickGeom {
   int32_t chunk;      // records per chunk
   int32_t packed;     // ray ids packed as t << 20 | i << 10 | j (else (t n1 + i) n2 + j)
   int32_t shuffle;    // records of a brick stored in a permuted order
+  int32_t sort;       // records of each workgroup pass sorted by their step counts
 };
 
 __device__ __forceinline__ uint32_t ray_id(const BrickGeom &g, uint32_t n1, uint32_t n2, int t, int i,
@@ -235,6 +236,78 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
 // formed in T exactly as the reference forms it.
 using Acc = double;
 
+// Chunk c -> its brick b and record range [r0, r1) (uniform per workgroup).
+__device__ __forceinline__ int chunk_range(const BrickGeom &g, const unsigned long long *offsets,
+                                           const unsigned long long *chunk_prefix,
+                                           unsigned long long c, unsigned long long &r0,
+                                           unsigned long long &r1) {
+  int lo = 0, hi = g.nbricks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (chunk_prefix[mid] <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  r0 = offsets[lo] + (c - chunk_prefix[lo]) * (unsigned long long)g.chunk;
+  r1 = r0 + g.chunk;
+  if (r1 > offsets[lo + 1]) r1 = offsets[lo + 1];
+  return lo;
+}
+
+// Plan order 2: in every pass of the backprojection workgroup over a chunk
+// (kBpjBlock consecutive records, one per lane), the records are reordered by
+// decreasing step count, so that each wavefront's 64 walks are about equally
+// long; the reordering stays inside the pass, whose records come from nearby
+// rays.  Stable (ties keep their trace order), so the plan is deterministic.
+template <typename T>
+__global__ __launch_bounds__(kBpjBlock) void k_plan_sort(BrickGeom g,
+                                                         const unsigned long long *offsets,
+                                                         const unsigned long long *chunk_prefix,
+                                                         RecPtrs<T> rec) {
+  __shared__ uint32_t key[kBpjBlock];
+  __shared__ uint32_t s_rid[kBpjBlock], s_lq[kBpjBlock];
+  __shared__ T s_d0[kBpjBlock], s_d1[kBpjBlock], s_d2[kBpjBlock];
+  unsigned long long r0, r1;
+  chunk_range(g, offsets, chunk_prefix, blockIdx.x, r0, r1);
+  const uint32_t tid = threadIdx.x;
+  for (unsigned long long ws = r0; ws < r1; ws += kBpjBlock) {
+    const unsigned long long q = ws + tid;
+    if (q < r1) {
+      s_rid[tid] = rec.rid[q];
+      s_lq[tid] = rec.lq[q];
+      s_d0[tid] = rec.d0[q];
+      s_d1[tid] = rec.d1[q];
+      s_d2[tid] = rec.d2[q];
+      key[tid] = ((0xffffu - (s_lq[tid] >> 16)) << 10) | tid;  // longest first, then trace order
+    } else {
+      key[tid] = 0xffffffffu;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= kBpjBlock; k <<= 1) {  // bitonic sort, ascending
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        const uint32_t o = tid ^ j;
+        if (o > tid) {
+          const uint32_t x = key[tid], y = key[o];
+          const bool up = (tid & k) == 0;
+          if ((x > y) == up) {
+            key[tid] = y;
+            key[o] = x;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (q < r1) {
+      const uint32_t src = key[tid] & (kBpjBlock - 1);
+      rec.rid[q] = s_rid[src];
+      rec.lq[q] = s_lq[src];
+      rec.d0[q] = s_d0[src];
+      rec.d1[q] = s_d1[src];
+      rec.d2[q] = s_d2[src];
+    }
+    __syncthreads();
+  }
+}
+
 // The work of one workgroup: chunk c of one brick's records, accumulated in
 // the LDS array acc and flushed to the cube (or to the float64 accumulator).
 template <typename T>
@@ -244,17 +317,8 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
                                                const RecPtrs<T> &rec, unsigned long long c,
                                                Acc *acc) {
   const int vol = g.b[0] * g.b[1] * g.b[2];
-  // chunk -> brick (binary search over the chunk prefix, uniform per workgroup)
-  int lo = 0, hi = g.nbricks - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (chunk_prefix[mid] <= c) lo = mid;
-    else hi = mid - 1;
-  }
-  const int b = lo;
-  const unsigned long long r0 = offsets[b] + (c - chunk_prefix[b]) * (unsigned long long)g.chunk;
-  unsigned long long r1 = r0 + g.chunk;
-  if (r1 > offsets[b + 1]) r1 = offsets[b + 1];
+  unsigned long long r0, r1;
+  const int b = chunk_range(g, offsets, chunk_prefix, c, r0, r1);
   const int nsp = g.nb[0] * g.nb[1] * g.nb[2];
   const int bin = b / nsp;
   const int bs = b % nsp;
@@ -377,6 +441,7 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
   g.chunk = p->chunk_records > 0 ? p->chunk_records : 32768;
   g.packed = (c->n1 <= 1024 && c->n2 <= 1024 && c->nv <= 4096) ? 1 : 0;
   g.shuffle = p->order == 1 ? 1 : 0;
+  g.sort = p->order == 2 ? 1 : 0;
   return TG_OK;
 }
 
@@ -536,7 +601,13 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
       hipLaunchKernelGGL((k_plan_fill<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, a, g,
                          offsets, cursor, mult, rp);
   }
-  return hip_check(hipGetLastError(), "plan fill launch");
+  if ((rc = hip_check(hipGetLastError(), "plan fill launch"))) return rc;
+  if (g.sort && p->chunks > 0) {
+    hipLaunchKernelGGL((k_plan_sort<T>), dim3((uint32_t)p->chunks), dim3(kBpjBlock), 0, s, g,
+                       offsets, (const unsigned long long *)(meta + m.chunk_prefix), rp);
+    if ((rc = hip_check(hipGetLastError(), "plan sort launch"))) return rc;
+  }
+  return TG_OK;
 }
 
 // Lets a kernel use up to 160 KiB of dynamic LDS (once per kernel).
@@ -666,9 +737,9 @@ extern "C" {
 int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p) {
   if (!c || !p) return set_error(TG_EINVAL, "null argument");
   if (p->brick[0] == 0 && p->brick[1] == 0 && p->brick[2] == 0) {
-    // default: 16 x 32 x 32 voxels = 128 KiB of double accumulators
-    p->brick[0] = 16;
-    p->brick[1] = 32;
+    // default: 32 x 16 x 32 voxels = 128 KiB of double accumulators
+    p->brick[0] = 32;
+    p->brick[1] = 16;
     p->brick[2] = 32;
   }
   BrickGeom g;

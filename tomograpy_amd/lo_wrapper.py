@@ -72,7 +72,7 @@ class Siddon:
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
                  voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
                  tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0,
-                 plan_order=0, plan_trace_tile=0, acc64=None, counted_forward=True):
+                 plan_order=2, plan_trace_tile=0, acc64=None, counted_forward=True):
         self.data_header = list(data_header)
         self.map_header = dict(cube_header)
         check_headers(self.data_header, self.map_header)

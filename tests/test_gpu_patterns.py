@@ -111,3 +111,42 @@ def test_admission_rejects_oversize_request():
     server = SV.Server(budget_bytes=1024, max_active=2)
     with pytest.raises(ValueError, match="budget"):
         server.run(reqs)
+
+
+def test_plan_batch_mixed_jobs_match_one_job_calls():
+    """One plan launch over jobs of different kinds (time bins, the Sun, Thomson
+    weights, float64 accumulation on and off) equals each job's own
+    backprojection: to the ulp with float64 sums, to 1e-5 without."""
+    from tomograpy_amd.lo_wrapper import Siddon4d
+    print("plan batch mixed", flush=True)
+    rng = np.random.default_rng(11)
+    ops = []
+    for k in range(5):
+        n = int(rng.integers(20, 48))
+        nv = int(rng.integers(4, 10))
+        npx = int(rng.choice([32, 48]))
+        ch = W.cube_header(n)
+        hs = W.view_headers(npx, rng.uniform(0, 2 * np.pi, nv), np.deg2rad(rng.uniform(-7, 7, nv)),
+                            d=float(rng.uniform(150, 250)))
+        if k == 0:
+            P = Siddon4d(hs, ch, tindex=np.arange(nv) % 3, dtype=np.float32, bpj_mode="plan")
+        elif k == 1:
+            P = Mo.thomson_lo(hs, ch, dtype=np.float32, bpj_mode="plan")
+        elif k == 2:
+            P = Siddon(hs, ch, dtype=np.float32, obstacle="sun", bpj_mode="plan", acc64=False)
+        else:
+            P = Siddon(hs, ch, dtype=np.float32, bpj_mode="plan", brick=(8, 16, 16))
+        ops.append(P)
+    ys = [torch.rand((P.nv, P.n1, P.n2), device="cuda").permute(1, 2, 0) for P in ops]
+    single = [P.rmatvec(y).clone() for P, y in zip(ops, ys)]
+    outs = [torch.zeros_like(s) for s in single]
+    B.run(ops, ys, outs, backward=True)
+    torch.cuda.synchronize()
+    for P, s, o in zip(ops, single, outs):
+        a, b = o.cpu().numpy(), s.cpu().numpy()
+        if P.acc64:
+            d = a != b
+            assert d.mean() <= 1e-3
+            assert np.all(np.abs(a - b)[d] <= np.spacing(np.maximum(np.abs(a), np.abs(b)))[d])
+        else:
+            assert rel_l2(a, b) <= 1e-5

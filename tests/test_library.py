@@ -195,3 +195,28 @@ def test_bpj_plan_init_fields(dtype_code, n, brick):
     assert p.meta_bytes >= p.nbricks * (4 + 8 + 4 + 8 + 4)
     p.brick[0] = 12
     assert N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)) == N.TG_EINVAL
+
+
+def test_plan_batch_rejects_bad_arguments_without_gpu():
+    """The plan batch validates its arguments before any device call."""
+    lib = N.lib()
+    assert lib.tg_bpj_plan_batch_bytes(4) > lib.tg_bpj_plan_batch_bytes(1) > 0
+    calls = (N.TgCall * 1)()
+    plans = (N.TgBpjPlan * 1)()
+    assert lib.tg_bpj_plan_batch(calls, plans, 0, None, None, None) == N.TG_EINVAL
+    assert b"empty" in lib.tg_last_error()
+    assert lib.tg_bpj_plan_batch(calls, plans, 1, None, None, None) == N.TG_EINVAL
+    assert b"staging" in lib.tg_last_error()
+    stage = np.zeros(int(lib.tg_bpj_plan_batch_bytes(1)), dtype=np.uint8)
+    calls[0].dtype = N.TG_F32
+    calls[0].kind = N.TG_PJ  # not a backprojection: rejected while the table is staged,
+    # before the table copy (the queue pointer is never used here)
+    rc = lib.tg_bpj_plan_batch(calls, plans, 1, stage.ctypes.data, stage.ctypes.data, None)
+    assert rc == N.TG_EINVAL
+    assert b"bpj" in lib.tg_last_error()
+
+
+def test_plan_trace_tile_field_follows_order():
+    """tg_bpj_plan.trace_tile sits between order and records, as in the header."""
+    assert N.TgBpjPlan.trace_tile.offset == N.TgBpjPlan.order.offset + 4
+    assert N.TgBpjPlan.records.offset == N.TgBpjPlan.trace_tile.offset + 4

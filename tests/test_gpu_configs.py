@@ -32,7 +32,7 @@ def test_c3_gradient_against_finite_differences_fp64():
     L.backward()
     g = x.grad.detach()
     h = 1e-3
-    rounding = 16 * np.finfo(np.float64).eps * abs(float(L)) / h
+    rounding = 16 * np.finfo(np.float64).eps * abs(float(L.detach())) / h
     rng = np.random.default_rng(31)
     idx = rng.integers(0, 128, size=(64, 3))
     bins = rng.integers(0, 8, size=64)

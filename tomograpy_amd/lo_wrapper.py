@@ -67,6 +67,18 @@ class Siddon:
     ``vox_weight`` (the diagonals O and M of the Thomson operator O.P.M,
     models.py:223-235), ``deterministic`` (fixed-order backprojection) and
     ``dtype``.
+
+    Execution options (results do not depend on them beyond summation order):
+
+    * ``bpj_mode``: "auto" (brick plan when the scatter would be contended),
+      "plan" or "atomic";
+    * ``brick``, ``chunk_records``, ``plan_order`` (2: passes sorted by step
+      count), ``plan_trace_tile``: the brick plan's shape, see tg_bpj_plan;
+    * ``acc64`` (default: on in float32): sum backprojections in float64, so
+      that they do not depend on the order of summation;
+    * ``counted_forward`` (default on): count each ray's steps once and run
+      the projector's loop for that count;
+    * ``tile_j``: pixels per row of a projector workgroup's ray tile.
     """
 
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,

@@ -327,20 +327,22 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
   for (int k = threadIdx.x; k < vol; k += blockDim.x) acc[k] = (Acc)0;
   __syncthreads();
   for (unsigned long long p = r0 + threadIdx.x; p < r1; p += blockDim.x) {
+    // the record's own fields in one round trip, then the two gathers that
+    // depend on its ray id (pixel value, ray constants) in a second one
+    const uint32_t rid = rec.rid[p];
+    const uint32_t lq = rec.lq[p];
+    T D0 = rec.d0[p], D1 = rec.d1[p], D2 = rec.d2[p];
     int t, i, j;
-    ray_pixel(g, (uint32_t)a.n1, (uint32_t)a.n2, rec.rid[p], t, i, j);
+    ray_pixel(g, (uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
     const int64_t did = a.data32 ? (int64_t)((uint32_t)i * (uint32_t)a.ds[0] +
                                              (uint32_t)j * (uint32_t)a.ds[1] +
                                              (uint32_t)t * (uint32_t)a.ds[2])
                                  : i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
     T w = a.data[did];
-    if (a.pixw) w = a.pixw[did] * w;
-    if (w == (T)0) continue;
-    // per-ray constants, the record's entry state and its step count
     const RayEntry<T> e =
         rec.ray[((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j];
-    T D0 = rec.d0[p], D1 = rec.d1[p], D2 = rec.d2[p];
-    const uint32_t lq = rec.lq[p];
+    if (a.pixw) w = a.pixw[did] * w;
+    if (w == (T)0) continue;
     // byte offsets in the double accumulators of one step along each axis
     const int u0 = ((e.up >> 0) & 3) - 1, u1 = ((e.up >> 2) & 3) - 1, u2 = ((e.up >> 4) & 3) - 1;
     const int dm0 = u0 * (int)(sizeof(Acc) << (g.lg[1] + g.lg[2]));

@@ -326,7 +326,13 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
   const int lo0 = bx << g.lg[0], lo1 = by << g.lg[1], lo2 = bz << g.lg[2];
   for (int k = threadIdx.x; k < vol; k += blockDim.x) acc[k] = (Acc)0;
   __syncthreads();
-  for (unsigned long long p = r0 + threadIdx.x; p < r1; p += blockDim.x) {
+  // The record loop is uniform per wavefront (every lane takes part in the
+  // ballot below); lanes past the chunk's end walk no steps.
+  const uint32_t lane = threadIdx.x & 63u;
+  const unsigned long long wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x - lane);
+  for (unsigned long long pw = r0 + wave0; pw < r1; pw += blockDim.x) {
+    const bool valid = pw + lane < r1;
+    const unsigned long long p = valid ? pw + lane : r1 - 1;
     // the record's own fields in one round trip, then the two gathers that
     // depend on its ray id (pixel value, ray constants) in a second one
     const uint32_t rid = rec.rid[p];
@@ -342,15 +348,16 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
     const RayEntry<T> e =
         rec.ray[((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j];
     if (a.pixw) w = a.pixw[did] * w;
-    if (w == (T)0) continue;
+    // a zero pixel value walks its steps adding zeros, which keeps the
+    // wavefront together and leaves every sum unchanged
     // byte offsets in the double accumulators of one step along each axis
     const int u0 = ((e.up >> 0) & 3) - 1, u1 = ((e.up >> 2) & 3) - 1, u2 = ((e.up >> 4) & 3) - 1;
     const int dm0 = u0 * (int)(sizeof(Acc) << (g.lg[1] + g.lg[2]));
     const int dm1 = u1 * (int)(sizeof(Acc) << g.lg[2]);
     const int dm2 = u2 * (int)sizeof(Acc);
     unsigned char *cell = (unsigned char *)(acc + (lq & 0xffffu));
-    const uint32_t n = lq >> 16;
-    for (uint32_t k = 0; k < n; k++) {
+    const uint32_t n = valid ? lq >> 16 : 0u;
+    auto step = [&]() {
       const T d = tmin(tmin(D0, D1), D2);
       const bool sx = (D0 == d), sy = (D1 == d), sz = (D2 == d);
       D0 = sx ? e.pabs[0] : D0 - d;
@@ -358,7 +365,14 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
       D2 = sz ? e.pabs[2] : D2 - d;
       atomicAdd((Acc *)cell, (Acc)(d * w));
       cell += (sx ? dm0 : 0) + (sy ? dm1 : 0) + (sz ? dm2 : 0);
-    }
+    };
+    // Steps that every lane walks run on a scalar counter.  In a plan sorted
+    // by decreasing step count the last lane holds the wavefront's fewest
+    // steps; the ballot checks that (any other order walks everything per lane).
+    uint32_t common = (uint32_t)__builtin_amdgcn_readlane((int)n, 63);
+    if (__ballot(n < common) != 0ull) common = 0;
+    for (uint32_t k = 0; k < common; k++) step();
+    for (uint32_t k = common; k < n; k++) step();
   }
   __syncthreads();
   const int64_t boff = (int64_t)bin * a.cs[3];

@@ -77,7 +77,8 @@ typedef struct tg_call {
   int32_t dtype;  /* TG_F32 | TG_F64 */
   int32_t kind;   /* TG_PJ .. TG_BPJT */
   uint32_t flags; /* TG_OBSTACLE_SUN | TG_DETERMINISTIC */
-  int32_t tile_j; /* pixels per block row (power of two, 1..256); 0 = default */
+  int32_t tile_j; /* pixels per block row (power of two, 1..256); 0 = n2 rounded up to a
+                     power of two, at most 256 */
   tg_map map;
   /* image stack, shape (n1, n2, nv) as in the reference (images on the last axis) */
   int32_t n1, n2, nv, pad0;

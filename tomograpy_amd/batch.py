@@ -16,7 +16,7 @@ import torch
 from . import _native as N
 from . import siddon as S
 
-TILE_CHOICES = (8, 16, 32, 64)
+TILE_CHOICES = (0, 16, 64, 256)  # 0: each job's default (its image width, at most 256)
 SCHEDULES = (0, 1)
 
 

@@ -182,7 +182,12 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a, int tile_j = 0) {
   a.t1 = t1;
   a.ray_i = -1;
   a.ray_j = -1;
-  int tj = tile_j > 0 ? tile_j : (c->tile_j > 0 ? c->tile_j : 16);
+  // Default tile: one workgroup spans up to 256 pixels of an image column
+  // (the image's second axis, rounded up to a power of two), so that its rays
+  // share one direction across the first axis and walk the same voxel columns.
+  int tj_default = 1;
+  while (tj_default < c->n2 && tj_default < 256) tj_default <<= 1;
+  int tj = tile_j > 0 ? tile_j : (c->tile_j > 0 ? c->tile_j : tj_default);
   if (tj > 256 || (tj & (tj - 1))) return set_error(TG_EINVAL, "tile_j must be a power of two <= 256");
   int lg = 0;
   while ((1 << lg) < tj) lg++;

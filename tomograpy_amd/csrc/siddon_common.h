@@ -258,6 +258,20 @@ constexpr int kNoStep = 1 << 22;
 __device__ __forceinline__ float tmin(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ double tmin(double a, double b) { return fmin(a, b); }
 
+// A value no larger than x on any active lane of the wavefront, found from the
+// candidate c: while some active lane holds less, c moves to the value of the
+// first such lane.  Every round lowers c, and it ends at the active lanes'
+// minimum when c starts at an active lane's value (or at c, when c is already
+// no larger).  c is wavefront-uniform, so loops bounded by the result run on
+// a scalar counter with every lane active.
+__device__ __forceinline__ uint32_t wave_floor(uint32_t x, uint32_t c) {
+  for (;;) {
+    const unsigned long long below = __ballot(x < c);
+    if (below == 0ull) return c;
+    c = (uint32_t)__builtin_amdgcn_readlane((int)x, (int)__builtin_ctzll(below));
+  }
+}
+
 template <bool FAST>
 struct Walk;
 

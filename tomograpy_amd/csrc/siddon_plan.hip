@@ -368,9 +368,8 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
     };
     // Steps that every lane walks run on a scalar counter.  In a plan sorted
     // by decreasing step count the last lane holds the wavefront's fewest
-    // steps; the ballot checks that (any other order walks everything per lane).
-    uint32_t common = (uint32_t)__builtin_amdgcn_readlane((int)n, 63);
-    if (__ballot(n < common) != 0ull) common = 0;
+    // steps, so the search starts there.
+    const uint32_t common = wave_floor(n, (uint32_t)__builtin_amdgcn_readlane((int)n, 63));
     for (uint32_t k = 0; k < common; k++) step();
     for (uint32_t k = common; k < n; k++) step();
   }

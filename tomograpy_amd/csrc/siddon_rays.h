@@ -73,45 +73,40 @@ __device__ __forceinline__ void project_tile(const KArgs<T> &a, uint32_t tile) {
         offv += (uint32_t)r.iv[k] * (uint32_t)a.vs[k];
       }
     }
+    // One step: `cur` is this voxel's value, `next` receives the next voxel's
+    // load, which is in flight during the following step.
+    auto step = [&](const T cur_in, T &next) {
+      T cur = cur_in;
+      bool keep = true;
+      if (VOX) {
+        if (a.voxw) cur = a.voxw[offv] * cur;
+        if (a.voxmask) keep = (a.voxmask[offv] == 0);
+      }
+      const T d = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
+      const bool sx = (r.D[0] == d), sy = (r.D[1] == d), sz = (r.D[2] == d);
+      r.D[0] = sx ? r.pabs[0] : r.D[0] - d;
+      r.D[1] = sy ? r.pabs[1] : r.D[1] - d;
+      r.D[2] = sz ? r.pabs[2] : r.D[2] - d;
+      off += (uint32_t)((sx ? dm[0] : 0) + (sy ? dm[1] : 0) + (sz ? dm[2] : 0));
+      if (VOX) offv += (uint32_t)((sx ? dv[0] : 0) + (sy ? dv[1] : 0) + (sz ? dv[2] : 0));
+      next = buffer_load<T>(rs, off);
+      if (keep) acc = acc + d * cur;
+    };
+    // Unrolled by two so that the two values alternate registers: no copy of a
+    // value whose load is still in flight.  The pairs of steps every lane of
+    // the wavefront walks run on a scalar counter, the rest per lane.
     T v0 = buffer_load<T>(rs, off), v1 = (T)0;
+    const uint32_t common = wave_floor(nsteps, __builtin_amdgcn_readfirstlane(nsteps)) & ~1u;
     uint32_t s = 0;
+    for (; s < common; s += 2) {
+      step(v0, v1);
+      step(v1, v0);
+    }
     while (true) {
       if (s >= nsteps) break;
-      {
-        T cur = v0;
-        bool keep = true;
-        if (VOX) {
-          if (a.voxw) cur = a.voxw[offv] * cur;
-          if (a.voxmask) keep = (a.voxmask[offv] == 0);
-        }
-        const T d = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
-        const bool sx = (r.D[0] == d), sy = (r.D[1] == d), sz = (r.D[2] == d);
-        r.D[0] = sx ? r.pabs[0] : r.D[0] - d;
-        r.D[1] = sy ? r.pabs[1] : r.D[1] - d;
-        r.D[2] = sz ? r.pabs[2] : r.D[2] - d;
-        off += (uint32_t)((sx ? dm[0] : 0) + (sy ? dm[1] : 0) + (sz ? dm[2] : 0));
-        if (VOX) offv += (uint32_t)((sx ? dv[0] : 0) + (sy ? dv[1] : 0) + (sz ? dv[2] : 0));
-        v1 = buffer_load<T>(rs, off);
-        if (keep) acc = acc + d * cur;
-      }
+      step(v0, v1);
       if (++s >= nsteps) break;
-      {
-        T cur = v1;
-        bool keep = true;
-        if (VOX) {
-          if (a.voxw) cur = a.voxw[offv] * cur;
-          if (a.voxmask) keep = (a.voxmask[offv] == 0);
-        }
-        const T d = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
-        const bool sx = (r.D[0] == d), sy = (r.D[1] == d), sz = (r.D[2] == d);
-        r.D[0] = sx ? r.pabs[0] : r.D[0] - d;
-        r.D[1] = sy ? r.pabs[1] : r.D[1] - d;
-        r.D[2] = sz ? r.pabs[2] : r.D[2] - d;
-        off += (uint32_t)((sx ? dm[0] : 0) + (sy ? dm[1] : 0) + (sz ? dm[2] : 0));
-        if (VOX) offv += (uint32_t)((sx ? dv[0] : 0) + (sy ? dv[1] : 0) + (sz ? dv[2] : 0));
-        v0 = buffer_load<T>(rs, off);
-        if (keep) acc = acc + d * cur;
-      }
+      step(v1, v0);
       ++s;
     }
   } else if (FAST && a.cube_bytes && in_map(a, r)) {

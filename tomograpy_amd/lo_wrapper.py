@@ -78,7 +78,8 @@ class Siddon:
       that they do not depend on the order of summation;
     * ``counted_forward`` (default on): count each ray's steps once and run
       the projector's loop for that count;
-    * ``tile_j``: pixels per row of a projector workgroup's ray tile.
+    * ``tile_j``: pixels per row of a projector workgroup's ray tile (0: the
+      image's second axis rounded up to a power of two, at most 256).
     """
 
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,

@@ -154,6 +154,15 @@ size_t tg_det_workspace_min_bytes(const tg_call *c);
  * voxel-weight half of the Thomson operator's adjoint (models.py:223-235). */
 int tg_scale_cube(const tg_call *c, void *stream);
 
+/* c->cube[v] = vox_weight[v] * src[v], or 0 where voxmask[v] != 0 (either
+ * operand may be NULL), over the call's cube shape and every time bin of the
+ * time kinds; src has its own element strides (src_stride[3] steps the bins)
+ * and must not be c->cube.  This is the diag(f) of the Thomson operator
+ * (models.py:223-235) and the zero-filled masked unknowns of mask_object
+ * (models.py:155-165) applied to the map before a forward projection, so that
+ * the projector then reads one value per Siddon step. */
+int tg_vox_apply(const tg_call *c, const void *src, const int64_t src_stride[4], void *stream);
+
 /* ---- brick-privatised backprojection (no reference counterpart; replaces the
  * atomic scatter of C:923-928 / C:939-944 for large cubes) ----
  * A plan, built once per geometry (views, map, mask, obstacle, time bins),
@@ -176,6 +185,12 @@ typedef struct tg_bpj_plan {
   int32_t trace_tile;    /* pixels per row of the ray tiles the plan build traces (power of
                             two); sets which rays are neighbours in a brick's record list,
                             hence in a wavefront of the backprojection; 0 = 256 */
+  int32_t tie_spread;    /* order 2: records with equal step counts keep the order of their
+                            pass positions with the low tie_spread bits reversed (0..10;
+                            0 = trace order).  Spreading them puts rays that are not
+                            neighbours in one wavefront, which pays when a voxel spans
+                            several pixels: adjacent rays then add into the same LDS cells */
+  int32_t pad;
   int64_t records;       /* set by tg_bpj_plan_count */
   int64_t chunks;        /* set by tg_bpj_plan_count */
   void *meta;            /* device, caller-owned, meta_bytes */

@@ -342,6 +342,30 @@ def test_acc64_backprojection_is_order_independent(oracle):
     assert float(Pp.acc_buffer().abs().max()) == 0.0
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("spread", [0, 3, 6, 10])
+def test_plan_tie_spread_only_reorders(oracle, spread):
+    """Plans whose sorted passes order ties differently (tg_bpj_plan.tie_spread)
+    give the same float64-summed cube, within 1e-5 of the oracle."""
+    print("plan tie spread", spread, flush=True)
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    ch, cube, data = c1_problem(np.float32)
+    rng = np.random.default_rng(29)
+    y = rng.uniform(-1, 1, data.shape).astype(np.float32)
+    ref = np.zeros(cube.shape, np.float32)
+    oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref,
+                  T.InfoArray(y, header=data.header), nthread=1)
+    yt = torch.from_numpy(y).cuda()
+    P0 = siddon_lo(data.header, ch, dtype=np.float32, bpj_mode="plan", plan_tie_spread=0)
+    Ps = siddon_lo(data.header, ch, dtype=np.float32, bpj_mode="plan", plan_tie_spread=spread)
+    o0 = P0.rmatvec(yt).cpu().numpy()
+    os_ = Ps.rmatvec(yt).cpu().numpy()
+    T.check_status()
+    assert Ps.build_plan().tie_spread == spread
+    assert rel_l2(os_, ref) <= 1e-5
+    assert_ulp_close(os_, o0)
+
+
 @pytest.mark.parametrize("n,brick,chunk", [(40, (16, 16, 16), 0), (24, (8, 16, 32), 100),
                                            (33, (32, 8, 8), 7)])
 def test_plan_backprojection_edges(oracle, n, brick, chunk):

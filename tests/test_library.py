@@ -195,6 +195,12 @@ def test_bpj_plan_init_fields(dtype_code, n, brick):
     assert p.meta_bytes >= p.nbricks * (4 + 8 + 4 + 8 + 4)
     p.brick[0] = 12
     assert N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)) == N.TG_EINVAL
+    p.brick[0] = b[0]
+    p.tie_spread = 11
+    assert N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)) == N.TG_EINVAL
+    assert b"tie_spread" in N.lib().tg_last_error()
+    p.tie_spread = 10
+    N.check(N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)))
 
 
 def test_plan_batch_rejects_bad_arguments_without_gpu():
@@ -216,7 +222,27 @@ def test_plan_batch_rejects_bad_arguments_without_gpu():
     assert b"bpj" in lib.tg_last_error()
 
 
-def test_plan_trace_tile_field_follows_order():
-    """tg_bpj_plan.trace_tile sits between order and records, as in the header."""
+def test_plan_trace_tile_and_tie_spread_fields_follow_order():
+    """tg_bpj_plan.trace_tile, tie_spread and pad sit between order and records,
+    as in the header."""
     assert N.TgBpjPlan.trace_tile.offset == N.TgBpjPlan.order.offset + 4
-    assert N.TgBpjPlan.records.offset == N.TgBpjPlan.trace_tile.offset + 4
+    assert N.TgBpjPlan.tie_spread.offset == N.TgBpjPlan.trace_tile.offset + 4
+    assert N.TgBpjPlan.records.offset == N.TgBpjPlan.tie_spread.offset + 8
+
+
+def test_auto_tie_spread_follows_the_voxel_to_pixel_ratio():
+    """Plans spread ties only when a voxel spans several pixels (about 1.9 pixels
+    per voxel at 256^3 seen in 512^2, 3.7 at 128^3, 8.7 at 55^3)."""
+    import types
+    from tomograpy_amd import workloads as W
+    from tomograpy_amd.headers import map_record
+    from tomograpy_amd.lo_wrapper import Siddon
+
+    def spread(n, npix):
+        ch, hs, _ = W.config2_geometry(np.float32, n=n, n_views=8, n_pix=npix)
+        return Siddon.auto_tie_spread(types.SimpleNamespace(data_header=hs, map_rec=map_record(ch)))
+
+    assert spread(256, 512) == 0
+    assert spread(128, 256) == 0
+    assert spread(128, 512) == 6
+    assert spread(55, 512) == 10

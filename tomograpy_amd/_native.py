@@ -59,7 +59,8 @@ class TgCgSeg(ctypes.Structure):
 
 class TgBpjPlan(ctypes.Structure):
     _fields_ = [("brick", _i32 * 3), ("chunk_records", _i32), ("nb_axis", _i32 * 3),
-                ("nbricks", _i32), ("order", _i32), ("trace_tile", _i32), ("records", _i64), ("chunks", _i64), ("meta", _vp),
+                ("nbricks", _i32), ("order", _i32), ("trace_tile", _i32),
+                ("tie_spread", _i32), ("pad", _i32), ("records", _i64), ("chunks", _i64), ("meta", _vp),
                 ("meta_bytes", _sz), ("rec", _vp), ("rec_bytes", _sz)]
 
 
@@ -82,6 +83,7 @@ SIGNATURES = [
     ("tg_count_steps", ctypes.c_int, [ctypes.POINTER(TgCall), _vp, _vp, _vp]),
     ("tg_det_workspace_min_bytes", _sz, [ctypes.POINTER(TgCall)]),
     ("tg_scale_cube", ctypes.c_int, [ctypes.POINTER(TgCall), _vp]),
+    ("tg_vox_apply", ctypes.c_int, [ctypes.POINTER(TgCall), _vp, ctypes.POINTER(_i64), _vp]),
     ("tg_regularize", ctypes.c_int, [_i32, ctypes.POINTER(_i32), _u32, _dbl, _i32, _vp, _vp, _vp, _vp]),
     ("tg_cg_dot", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _i32, _vp]),
     ("tg_cg_update", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),

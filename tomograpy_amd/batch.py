@@ -102,6 +102,7 @@ def run(ops, ins, outs, backward=False, plan=None, zero_out=False):
     post = []
     copies = []
     alive = []  # temporaries the launches read: referenced until they are enqueued
+    weighted = set()
     for k, (P, x, y) in enumerate(zip(ops, ins, outs)):
         if backward:
             target = y
@@ -119,7 +120,14 @@ def run(ops, ins, outs, backward=False, plan=None, zero_out=False):
                 copies.append((target, y))
             cube = P._as_cube(x)
             alive.append(cube)
-            c = P._call(N.TG_PJ, cube, target)
+            vox_out = None
+            if P.vox_weight is not None or P.voxmask is not None:
+                # an operator's weighted-map buffer serves its first job only
+                if id(P) in weighted:
+                    vox_out = torch.empty_like(P.xin)
+                    alive.append(vox_out)
+                weighted.add(id(P))
+            c = P._call(N.TG_PJ, cube, target, vox_out=vox_out)
             if zero_out:
                 c.flags |= N.TG_ZERO_OUT
         ctypes.memmove(ctypes.byref(calls[k]), ctypes.byref(c), ctypes.sizeof(N.TgCall))

@@ -131,6 +131,28 @@ __global__ __launch_bounds__(256) void k_scale_cube(KArgs<T> a) {
   }
 }
 
+// cube[v] = vox_weight[v] * src[v], or 0 where voxmask[v] != 0, over every
+// time bin: the voxel operand of a forward projection applied once per call
+// instead of once per Siddon step.
+template <typename T>
+__global__ __launch_bounds__(256) void k_vox_apply(KArgs<T> a, const T *__restrict__ src,
+                                                   int64_t s0, int64_t s1, int64_t s2, int64_t s3) {
+  const int64_t nvox = (int64_t)a.n[0] * a.n[1] * a.n[2];
+  const int64_t total = nvox * a.nt;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = k / nvox, v = k % nvox;
+    const int64_t i0 = v / ((int64_t)a.n[1] * a.n[2]);
+    const int64_t i1 = (v / a.n[2]) % a.n[1];
+    const int64_t i2 = v % a.n[2];
+    const int64_t m = i0 * a.vs[0] + i1 * a.vs[1] + i2 * a.vs[2];
+    T x = src[i0 * s0 + i1 * s1 + i2 * s2 + b * s3];
+    if (a.voxw) x = a.voxw[m] * x;
+    if (a.voxmask && a.voxmask[m] != 0) x = (T)0;
+    a.cube[i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2] + b * a.cs[3]] = x;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -385,6 +407,23 @@ int run_scale(const tg_call *c, hipStream_t s) {
   return hip_check(hipGetLastError(), "scale kernel launch");
 }
 
+template <typename T>
+int run_vox_apply(const tg_call *c, const void *src, const int64_t src_stride[4], hipStream_t s) {
+  KArgs<T> a;
+  tg_call cc = *c;
+  if (cc.kind != TG_PJT && cc.kind != TG_BPJT) cc.map.nt = 1;
+  int rc = make_args<T>(&cc, 0, cc.nv, a);
+  if (rc) return rc;
+  if (!c->cube || !src || !src_stride) return set_error(TG_EINVAL, "vox_apply needs cube and src");
+  if (src == (const void *)c->cube) return set_error(TG_EINVAL, "vox_apply: src and cube must differ");
+  const int64_t total = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.nt;
+  if (total == 0) return TG_OK;
+  const uint32_t blocks = (uint32_t)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
+  hipLaunchKernelGGL((k_vox_apply<T>), dim3(blocks), dim3(256), 0, s, a, (const T *)src,
+                     src_stride[0], src_stride[1], src_stride[2], src_stride[3]);
+  return hip_check(hipGetLastError(), "vox_apply kernel launch");
+}
+
 // ---------------------------------------------------------------------------
 // View packing (host): C:1043-1046 pixel -> angle in T, then libm cos/sin in
 // double as C:1015-1017 calls them.
@@ -496,6 +535,14 @@ int tg_scale_cube(const tg_call *c, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   if (c->dtype == TG_F32) return run_scale<float>(c, s);
   if (c->dtype == TG_F64) return run_scale<double>(c, s);
+  return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
+}
+
+int tg_vox_apply(const tg_call *c, const void *src, const int64_t src_stride[4], void *stream) {
+  if (!c) return set_error(TG_EINVAL, "null call");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->dtype == TG_F32) return run_vox_apply<float>(c, src, src_stride, s);
+  if (c->dtype == TG_F64) return run_vox_apply<double>(c, src, src_stride, s);
   return set_error(TG_EINVAL, "bad dtype %d", c->dtype);
 }
 

@@ -36,6 +36,7 @@ namespace {
 constexpr int kPlanBlock = 256;
 constexpr int kHistMax = 8192;  // bricks x bins histogrammed in LDS; above: global atomics
 constexpr int kBpjBlock = 1024;
+static_assert(kBpjBlock == 1024, "the plan sort keys hold a 10-bit pass position");
 constexpr size_t kPlanQueueHeader = 256;
 // Default pixels per row of the ray tiles traced by the plan build: one image
 // row of 256 pixels per workgroup, so that the 64 records a wavefront of the
@@ -54,6 +55,7 @@ struct BrickGeom {
   int32_t packed;     // ray ids packed as t << 20 | i << 10 | j (else (t n1 + i) n2 + j)
   int32_t shuffle;    // records of a brick stored in a permuted order
   int32_t sort;       // records of each workgroup pass sorted by their step counts
+  int32_t spread;     // low bits of the pass position reversed to order ties
 };
 
 __device__ __forceinline__ uint32_t ray_id(const BrickGeom &g, uint32_t n1, uint32_t n2, int t, int i,
@@ -257,7 +259,15 @@ __device__ __forceinline__ int chunk_range(const BrickGeom &g, const unsigned lo
 // (kBpjBlock consecutive records, one per lane), the records are reordered by
 // decreasing step count, so that each wavefront's 64 walks are about equally
 // long; the reordering stays inside the pass, whose records come from nearby
-// rays.  Stable (ties keep their trace order), so the plan is deterministic.
+// rays.  Ties are ordered by the pass position with its low `spread` bits
+// reversed (tg_bpj_plan.tie_spread), so the plan is deterministic.
+// x with its low s bits reversed (an involution on [0, 1024) for s <= 10).
+__device__ __forceinline__ uint32_t tie_pos(uint32_t x, int s) {
+  if (s <= 0) return x;
+  const uint32_t m = (1u << s) - 1u;
+  return (x & ~m) | (__brev(x & m) >> (32 - s));
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBpjBlock) void k_plan_sort(BrickGeom g,
                                                          const unsigned long long *offsets,
@@ -277,7 +287,11 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_sort(BrickGeom g,
       s_d0[tid] = rec.d0[q];
       s_d1[tid] = rec.d1[q];
       s_d2[tid] = rec.d2[q];
-      key[tid] = ((0xffffu - (s_lq[tid] >> 16)) << 10) | tid;  // longest first, then trace order
+      // longest first; ties by pass position with its low g.spread bits
+      // reversed: spread, the walks a wavefront takes together come from rays
+      // apart in the pass rather than from adjacent pixels, which in a coarse
+      // cube step through the same voxels and serialise on the same LDS cells
+      key[tid] = ((0xffffu - (s_lq[tid] >> 16)) << 10) | tie_pos(tid, g.spread);
     } else {
       key[tid] = 0xffffffffu;
     }
@@ -297,7 +311,7 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_sort(BrickGeom g,
       }
     }
     if (q < r1) {
-      const uint32_t src = key[tid] & (kBpjBlock - 1);
+      const uint32_t src = tie_pos(key[tid] & (kBpjBlock - 1), g.spread);
       rec.rid[q] = s_rid[src];
       rec.lq[q] = s_lq[src];
       rec.d0[q] = s_d0[src];
@@ -457,6 +471,9 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
   g.packed = (c->n1 <= 1024 && c->n2 <= 1024 && c->nv <= 4096) ? 1 : 0;
   g.shuffle = p->order == 1 ? 1 : 0;
   g.sort = p->order == 2 ? 1 : 0;
+  if (p->tie_spread < 0 || p->tie_spread > 10)
+    return set_error(TG_EINVAL, "tie_spread %d outside [0, 10]", p->tie_spread);
+  g.spread = p->tie_spread;
   return TG_OK;
 }
 

@@ -73,7 +73,8 @@ class Siddon:
     * ``bpj_mode``: "auto" (brick plan when the scatter would be contended),
       "plan" or "atomic";
     * ``brick``, ``chunk_records``, ``plan_order`` (2: passes sorted by step
-      count), ``plan_trace_tile``: the brick plan's shape, see tg_bpj_plan;
+      count), ``plan_trace_tile``, ``plan_tie_spread`` (None: from the
+      voxel/pixel size ratio): the brick plan's shape, see tg_bpj_plan;
     * ``acc64`` (default: on in float32): sum backprojections in float64, so
       that they do not depend on the order of summation;
     * ``counted_forward`` (default on): count each ray's steps once and run
@@ -85,7 +86,8 @@ class Siddon:
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
                  voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
                  tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0,
-                 plan_order=2, plan_trace_tile=0, acc64=None, counted_forward=True):
+                 plan_order=2, plan_trace_tile=0, plan_tie_spread=None, acc64=None,
+                 counted_forward=True):
         self.data_header = list(data_header)
         self.map_header = dict(cube_header)
         check_headers(self.data_header, self.map_header)
@@ -134,6 +136,7 @@ class Siddon:
         self.chunk_records = int(chunk_records)
         self.plan_order = int(plan_order)
         self.plan_trace_tile = int(plan_trace_tile)
+        self.plan_tie_spread = None if plan_tie_spread is None else int(plan_tie_spread)
         self._plan = None
         self._plan_bufs = None
         self._plan_count = None
@@ -145,14 +148,31 @@ class Siddon:
         # projections run each ray for its precounted number of steps
         self.counted_forward = bool(counted_forward)
         self._ray_steps = None
+        self._xw = None  # voxel-weighted map read by forward projections
 
     # -- brick-privatised backprojection plan ------------------------------
-    # In "auto" mode the plan is used when an atomic scatter would update each
-    # voxel (of every time bin) more than this many times on average, and the
-    # geometry has at least PLAN_MIN_STEPS steps: below that the scatter's
-    # contention is low and the plan's build and per-chunk flush do not pay.
-    PLAN_MIN_STEPS_PER_VOXEL = 16
+    # In "auto" mode the plan is used when the geometry has at least
+    # PLAN_MIN_STEPS steps (and PLAN_MIN_STEPS_PER_VOXEL steps per voxel of
+    # every time bin).  The brick kernel beats the atomic scatter even for
+    # sparse geometries (256^3 seen by 8 views of 64^2, 0.6 steps per voxel:
+    # 0.17 against 0.39 ms), and a build of a plan that small takes about 1 ms.
+    PLAN_MIN_STEPS_PER_VOXEL = 0
     PLAN_MIN_STEPS = 1 << 20
+
+    def auto_tie_spread(self):
+        """tg_bpj_plan.tie_spread for this geometry, from the ratio of the voxel
+        edge to a pixel's footprint at the observer's distance (median over the
+        views).  Under 3 the rays a wavefront walks together share few voxels
+        per step and trace order keeps their gathers close; above, rays from
+        adjacent pixels add into the same LDS cells and spreading them pays
+        (55^3 seen in 512^2 pixels, ratio 8.7: 2.36 -> 1.19 ms; 128^3, ratio
+        3.7: 2.52 -> 2.05 ms with 6 bits; 256^3, ratio 1.9: 6 bits cost 8%)."""
+        rec = view_records(self.data_header)
+        dist = np.linalg.norm(rec[:, 15:18], axis=1)
+        pix = float(np.median(dist * np.abs(rec[:, 3])))
+        vox = float(min(abs(v) for v in self.map_rec[0:3]))
+        ratio = vox / pix if pix > 0 else 0.0
+        return 0 if ratio < 3.0 else (6 if ratio < 4.5 else 10)
 
     def use_plan(self):
         """Whether backprojections go through the brick plan."""
@@ -181,6 +201,8 @@ class Siddon:
             p.chunk_records = self.chunk_records
             p.order = self.plan_order
             p.trace_tile = self.plan_trace_tile
+            p.tie_spread = (self.auto_tie_spread() if self.plan_tie_spread is None
+                            else self.plan_tie_spread)
             N.check(lib.tg_bpj_plan_init(ctypes.byref(c), ctypes.byref(p)))
             meta = torch.empty(max(int(p.meta_bytes), 1), dtype=torch.uint8, device=self.dev)
             p.meta = meta.data_ptr()
@@ -219,15 +241,26 @@ class Siddon:
         return 0 if self._plan_bufs is None else sum(b.numel() for b in self._plan_bufs)
 
     # -- calls -------------------------------------------------------------
-    def _call(self, kind, cube, data, counted=True):
+    def _call(self, kind, cube, data, counted=True, apply_vox=True, vox_out=None):
+        """The TgCall of one projection of `cube` into `data` (or back).
+
+        A forward projection with voxel operands (weights, mask) first writes
+        f * cube, zero at masked voxels, into the operator's buffer (or
+        `vox_out`) on the current stream, and the call reads that map: the
+        projector then loads one value per Siddon step instead of three.
+        """
         flags = self.flags
         if kind in (N.TG_BPJ, N.TG_BPJT) and self.deterministic:
             flags |= N.TG_DETERMINISTIC
-        vw = self.vox_weight if kind in (N.TG_PJ, N.TG_PJT) else None
+        vm = self.voxmask
+        if kind in (N.TG_PJ, N.TG_PJT):
+            vm = None
+            if apply_vox and (self.vox_weight is not None or self.voxmask is not None):
+                cube = self.vox_apply(cube, vox_out)
         c = S.make_call(kind, self.code, data.shape, data.stride(), cube.shape, cube.stride(),
                         self.map_rec, self.views, cube.data_ptr(), data.data_ptr(), flags=flags,
                         tile_j=self.tile_j, pixmask=self.pixmask, pix_weight=self.pix_weight,
-                        vox_weight=vw, voxmask=self.voxmask, status=self.status)
+                        voxmask=vm, status=self.status)
         if flags & N.TG_DETERMINISTIC:
             if self._ws is None:
                 self._ws = S.det_workspace(c, self.dev)
@@ -240,11 +273,29 @@ class Siddon:
             c.ray_steps = self.ray_steps().data_ptr()
         return c
 
+    def vox_apply(self, cube, out=None):
+        """out = f * cube with masked voxels zeroed (models.py:223-235 and
+        155-165), launched on the current stream; `out` defaults to the
+        operator's buffer.  `cube` has the operator's input shape."""
+        if out is None:
+            if self._xw is None:
+                self._xw = torch.empty_like(self.xin)
+            out = self._xw
+        time = cube.dim() == 4
+        c = S.make_call(N.TG_PJT if time else N.TG_PJ, self.code, self.xout.shape,
+                        self.xout.stride(), out.shape, out.stride(), self.map_rec, self.views,
+                        out.data_ptr(), self.xout.data_ptr(), nt=cube.shape[3] if time else 1,
+                        vox_weight=self.vox_weight, voxmask=self.voxmask)
+        st = (ctypes.c_int64 * 4)(*(list(cube.stride()) + [0] * (4 - cube.dim())))
+        N.check(N.lib().tg_vox_apply(ctypes.byref(c), ctypes.c_void_p(cube.data_ptr()), st,
+                                     S.stream_handle()))
+        return out
+
     def ray_steps(self):
         """Per-ray Siddon step counts of this geometry (device uint32, indexed
         (t * n1 + i) * n2 + j), computed once and reused by every projection."""
         if self._ray_steps is None:
-            c = self._call(N.TG_PJ, self.xin, self.xout, counted=False)
+            c = self._call(N.TG_PJ, self.xin, self.xout, counted=False, apply_vox=False)
             per = torch.zeros(self.nv * self.n1 * self.n2, dtype=torch.int32, device=self.dev)
             N.check(N.lib().tg_count_steps(ctypes.byref(c), ctypes.c_void_p(per.data_ptr()), None,
                                            S.stream_handle()))
@@ -379,9 +430,9 @@ class Siddon4d(Siddon):
         self.shapein = self.cube_shape + (self.nt,)
         self.xin = map4_buffer(*self.cube_shape, self.nt, self.tdtype, self.dev)
 
-    def _call(self, kind, cube, data, counted=True):
+    def _call(self, kind, cube, data, counted=True, apply_vox=True, vox_out=None):
         kind = {N.TG_PJ: N.TG_PJT, N.TG_BPJ: N.TG_BPJT}.get(kind, kind)
-        c = super()._call(kind, cube, data, counted)
+        c = super()._call(kind, cube, data, counted, apply_vox, vox_out)
         c.map.nt = self.nt
         c.tindex = self.tindex.data_ptr()
         return c

@@ -248,6 +248,17 @@ int tg_cg_update(int32_t dtype, int64_t n, void *x, void *r, const void *p, cons
                  double *scal, void *stream);
 int tg_cg_direction(int32_t dtype, int64_t n, const void *r, void *p, double *scal, void *stream);
 
+/* The smoothness priors of the regularised models (lo.diff, models.py:85-94,
+ * and the temporal term of stsrt, models.py:140) in differentiable form, over
+ * a C-contiguous array of shape[4] (trailing 1s for fewer axes):
+ *   scal[slot] = sum_k weights[k] * sum_m (x[m + e_k] - x[m])^2
+ * (the numpy.diff convention; squares and sums in double, in a fixed order)
+ * and, when grad is not NULL,
+ *   grad = 2 sum_k weights[k] D_k^T D_k x.
+ * Axes with weights[k] == 0 are skipped; scal as for tg_cg_dot. */
+int tg_smoothness(int32_t dtype, const int32_t shape[4], const double weights[4], const void *x,
+                  void *grad, double *scal, int32_t slot, void *stream);
+
 /* ---- segmented CG: the vector work of many concurrent solves in one launch
  * (no reference counterpart; used by the reconstruction server) ---- */
 

@@ -246,3 +246,19 @@ def test_auto_tie_spread_follows_the_voxel_to_pixel_ratio():
     assert spread(128, 256) == 0
     assert spread(128, 512) == 6
     assert spread(55, 512) == 10
+
+
+def test_smoothness_rejects_bad_arguments_without_gpu():
+    """tg_smoothness validates its arguments before any device call."""
+    lib = N.lib()
+    shape = (ctypes.c_int32 * 4)(4, 4, 1, 1)
+    w = (ctypes.c_double * 4)(1.0, 1.0, 0.0, 0.0)
+    assert lib.tg_smoothness(N.TG_F32, shape, w, None, None, None, 0, None) == N.TG_EINVAL
+    assert b"tg_smoothness" in lib.tg_last_error()
+    bad = (ctypes.c_int32 * 4)(4, 0, 1, 1)
+    x = np.zeros(16, np.float32)
+    scal = np.zeros(N.TG_CG_SCAL, np.float64)
+    assert lib.tg_smoothness(N.TG_F32, bad, w, x.ctypes.data, None, scal.ctypes.data, 0,
+                             None) == N.TG_EINVAL
+    assert lib.tg_smoothness(N.TG_F32, shape, w, x.ctypes.data, None, scal.ctypes.data, 16,
+                             None) == N.TG_EINVAL

@@ -88,6 +88,8 @@ SIGNATURES = [
     ("tg_cg_dot", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _i32, _vp]),
     ("tg_cg_update", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("tg_cg_direction", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp]),
+    ("tg_smoothness", ctypes.c_int, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_dbl), _vp, _vp, _vp,
+                                     _i32, _vp]),
     ("tg_bpj_plan_init", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan)]),
     ("tg_bpj_plan_count", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan), _vp]),
     ("tg_bpj_plan_build", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan), _vp]),

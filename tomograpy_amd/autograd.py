@@ -5,9 +5,17 @@ the (masked, time-binned, Thomson-weighted -- whatever P carries) Siddon
 backprojector: grad_x = P^T grad_y.  P is a ``lo_wrapper.Siddon`` or
 ``Siddon4d`` operator; the output is a fresh tensor (not P's cached buffer),
 so several projections can live in one autograd graph.
+
+``smoothness(x, weights)`` is the finite-difference prior of the regularised
+models (lo.diff, models.py:85-94; the temporal term of stsrt, models.py:140)
+as one differentiable HIP pass.
 """
+import ctypes
+
 import torch
 
+from . import _native as N
+from . import siddon as S
 from .lo_wrapper import stack_buffer
 
 
@@ -55,3 +63,39 @@ class SiddonBackproject(torch.autograd.Function):
 def backproject(y, P):
     """Differentiable backprojection of y through operator P."""
     return SiddonBackproject.apply(y, P)
+
+
+class Smoothness(torch.autograd.Function):
+    """sum_k w_k ||D_k x||^2 (D_k: numpy.diff along axis k) with the gradient
+    2 sum_k w_k D_k^T D_k x, both from one pass of ``tg_smoothness``."""
+
+    @staticmethod
+    def forward(ctx, x, weights):
+        if x.dim() > 4:
+            raise ValueError("smoothness takes arrays of up to 4 axes")
+        if x.dtype not in (torch.float32, torch.float64):
+            raise ValueError("smoothness takes float32 or float64 arrays")
+        w = [float(v) for v in weights]
+        if len(w) != x.dim():
+            raise ValueError("one weight per axis: %d weights for %d axes" % (len(w), x.dim()))
+        xc = x.detach().contiguous()
+        shape = (ctypes.c_int32 * 4)(*(list(xc.shape) + [1] * (4 - xc.dim())))
+        wts = (ctypes.c_double * 4)(*(w + [0.0] * (4 - len(w))))
+        grad = torch.empty_like(xc)
+        scal = torch.empty(N.TG_CG_SCAL, dtype=torch.float64, device=xc.device)
+        code = N.TG_F32 if xc.dtype == torch.float32 else N.TG_F64
+        N.check(N.lib().tg_smoothness(code, shape, wts, ctypes.c_void_p(xc.data_ptr()),
+                                      ctypes.c_void_p(grad.data_ptr()),
+                                      ctypes.c_void_p(scal.data_ptr()), 0, S.stream_handle()))
+        ctx.save_for_backward(grad)
+        return scal[0].to(x.dtype)
+
+    @staticmethod
+    def backward(ctx, gout):
+        (grad,) = ctx.saved_tensors
+        return grad * gout, None
+
+
+def smoothness(x, weights):
+    """Differentiable sum_k weights[k] ||D_k x||^2 over the axes of x."""
+    return Smoothness.apply(x, tuple(weights))

@@ -45,6 +45,48 @@ __global__ __launch_bounds__(256) void k_regularize(int64_t n0, int64_t n1, int6
   }
 }
 
+// Smoothness prior sum_k w_k ||D_k x||^2 and its gradient 2 sum_k w_k D_k^T D_k x
+// in one pass: per-block partial sums of the value (double) at
+// scal[kScalPartials + blockIdx.x], the gradient (in T) at grad (if not null).
+template <typename T>
+__global__ __launch_bounds__(256) void k_smooth(int64_t n0, int64_t n1, int64_t n2, int64_t n3,
+                                                double w0, double w1, double w2, double w3,
+                                                const T *__restrict__ x, T *__restrict__ grad,
+                                                double *scal) {
+  __shared__ double lds[256];
+  const int64_t total = n0 * n1 * n2 * n3;
+  const int64_t s3 = 1, s2 = n3, s1 = n2 * n3, s0 = n1 * n2 * n3;
+  const double w[4] = {w0, w1, w2, w3};
+  double v = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m[4] = {k / s0, (k / s1) % n1, (k / s2) % n2, k % n3};
+    const int64_t n[4] = {n0, n1, n2, n3};
+    const int64_t st[4] = {s0, s1, s2, s3};
+    const T xc = x[k];
+    T g = (T)0;
+#pragma unroll
+    for (int ax = 0; ax < 4; ax++) {
+      if (w[ax] == 0.0) continue;
+      const T wt = (T)w[ax];
+      if (m[ax] <= n[ax] - 2) {  // the difference x[m + e] - x[m] belongs to m
+        const T d = x[k + st[ax]] - xc;
+        v += w[ax] * ((double)d * (double)d);
+        g -= wt * d;
+      }
+      if (m[ax] >= 1) g += wt * (xc - x[k - st[ax]]);
+    }
+    if (grad) grad[k] = (T)2 * g;
+  }
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scal[kScalPartials + blockIdx.x] = lds[0];
+}
+
 // Per-block partial sums of a*b (double), written to partials[blockIdx.x].
 template <typename T>
 __device__ void block_dot_partial(int64_t n, const T *a, const T *b, double *partials) {
@@ -299,6 +341,29 @@ int tg_regularize(int32_t dtype, const int32_t shape[4], uint32_t axes_mask, dou
   else
     return set_error(TG_EINVAL, "bad dtype %d", dtype);
   return hip_check(hipGetLastError(), "regularize launch");
+}
+
+int tg_smoothness(int32_t dtype, const int32_t shape[4], const double weights[4], const void *x,
+                  void *grad, double *scal, int32_t slot, void *stream) {
+  if (!shape || !weights || !x || !scal || slot < 0 || slot >= 16)
+    return set_error(TG_EINVAL, "bad tg_smoothness arguments");
+  for (int k = 0; k < 4; k++)
+    if (shape[k] <= 0) return set_error(TG_EINVAL, "shape[%d] = %d must be positive", k, shape[k]);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = (int64_t)shape[0] * shape[1] * shape[2] * shape[3];
+  const uint32_t nb = red_blocks(n);
+  if (dtype == TG_F32)
+    hipLaunchKernelGGL((k_smooth<float>), dim3(nb), dim3(256), 0, s, shape[0], shape[1], shape[2],
+                       shape[3], weights[0], weights[1], weights[2], weights[3], (const float *)x,
+                       (float *)grad, scal);
+  else if (dtype == TG_F64)
+    hipLaunchKernelGGL((k_smooth<double>), dim3(nb), dim3(256), 0, s, shape[0], shape[1],
+                       shape[2], shape[3], weights[0], weights[1], weights[2], weights[3],
+                       (const double *)x, (double *)grad, scal);
+  else
+    return set_error(TG_EINVAL, "bad dtype %d", dtype);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s, scal, (int)nb, slot);
+  return hip_check(hipGetLastError(), "smoothness launch");
 }
 
 int tg_cg_dot(int32_t dtype, int64_t n, const void *a, const void *b, double *scal, int32_t slot,

@@ -188,7 +188,20 @@ def config3_problem(dtype=np.float32, **sizes):
 
 
 def config3_loss(P, y, lam=1e-2, lam_t=1e-2):
-    """x -> ||P x - y||^2 + lam ||grad_xyz x||^2 + lam_t ||d x / dt||^2 (differentiable)."""
+    """x -> ||P x - y||^2 + lam ||grad_xyz x||^2 + lam_t ||d x / dt||^2 (differentiable;
+    the two priors in one fused pass)."""
+    from .autograd import project, smoothness
+
+    def loss(x):
+        r = project(x, P) - y
+        return (r * r).sum() + smoothness(x, (lam, lam, lam, lam_t))
+
+    return loss
+
+
+def config3_loss_torch(P, y, lam=1e-2, lam_t=1e-2):
+    """config3_loss with the priors written as torch.diff terms (a check of the
+    fused prior)."""
     import torch
     from .autograd import project
 

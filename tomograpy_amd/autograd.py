@@ -32,7 +32,9 @@ class SiddonProject(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gy):
         P = ctx.P
-        gx = torch.zeros(P.shapein, dtype=P.tdtype, device=P.dev)
+        # the operator's own layout (bin-major for time bins): the brick plan then
+        # sums in float64, independent of the order of summation
+        gx = P.zeros_in()
         P.backproject_into(gy.contiguous() if gy.stride() != P.xout.stride() else gy, gx)
         return gx, None
 
@@ -48,7 +50,7 @@ class SiddonBackproject(torch.autograd.Function):
     @staticmethod
     def forward(ctx, y, P):
         ctx.P = P
-        x = torch.zeros(P.shapein, dtype=P.tdtype, device=P.dev)
+        x = P.zeros_in()
         P.backproject_into(y.detach(), x)
         return x
 

@@ -273,6 +273,11 @@ class Siddon:
             c.ray_steps = self.ray_steps().data_ptr()
         return c
 
+    def zeros_in(self):
+        """A zeroed map in this operator's storage layout (dense per time bin, as
+        the float64 backprojection accumulator needs)."""
+        return torch.zeros_like(self.xin)
+
     def vox_apply(self, cube, out=None):
         """out = f * cube with masked voxels zeroed (models.py:223-235 and
         155-165), launched on the current stream; `out` defaults to the

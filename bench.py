@@ -172,18 +172,28 @@ def batch_leg():
             "steps_per_s_batched": 2 * steps / (ms_b * 1e-3)}
 
 
-def serving_leg(args):
+def serving_leg(args, small=False):
     """The mixed-size reconstruction stream (32^3-256^3 cubes, 8-128 views of
     64^2-512^2, pixel mask on/off, 1-8 time bins, CG to a relative residual of
     1e-4), solved once by the continuous-batching server and once one request
-    at a time in arrival order; same requests, same seeds."""
+    at a time in arrival order; same requests, same seeds.  small=True: twice
+    as many requests from the small end of the mix (32^3-64^3 cubes, 8-24 views
+    of 64^2-128^2, 1-4 bins) with 32 slots, where a request alone does not fill
+    the GPU."""
     import torch
     from tomograpy_amd import serve as SV
     from tomograpy_amd import workloads as W
 
+    n_req = 2 * args.serve_requests if small else args.serve_requests
+    slots = 32 if small else args.serve_max_active
+
     def requests():
-        rng = np.random.default_rng(W.SEED0 + 5)
-        mix = W.config5_mix(args.serve_requests, rng)
+        rng = np.random.default_rng(W.SEED0 + (6 if small else 5))
+        if small:
+            mix = W.config5_mix(n_req, rng, n_range=(32, 64), v_range=(8, 24), pix=(64, 128),
+                                max_bins=4)
+        else:
+            mix = W.config5_mix(n_req, rng)
         return SV.make_requests(mix, rng, rate=args.serve_rate or None, tol=1e-4, maxiter=200)
 
     plain_reqs = requests()
@@ -192,12 +202,12 @@ def serving_leg(args):
     print("serving: %d requests made" % len(plain_reqs), file=sys.stderr, flush=True)
     plain = SV.serve_plain(plain_reqs)
     print("serving: plain done in %.1f s" % plain["makespan_s"], file=sys.stderr, flush=True)
-    served = SV.Server(budget_bytes=128 << 30, max_active=args.serve_max_active).run(served_reqs)
+    served = SV.Server(budget_bytes=128 << 30, max_active=slots).run(served_reqs)
     print("serving: served done in %.1f s" % served["makespan_s"], file=sys.stderr, flush=True)
     err = max(float(np.linalg.norm(a.x - b.x) / max(np.linalg.norm(b.x), 1e-300))
               for a, b in zip(served_reqs, plain_reqs))
-    return {"requests": len(plain_reqs), "arrival_rate_rps": args.serve_rate or None,
-            "max_active": args.serve_max_active, "tol": 1e-4,
+    out = {"requests": len(plain_reqs), "arrival_rate_rps": args.serve_rate or None,
+            "max_active": slots, "tol": 1e-4,
             "served": served, "plain": plain,
             "throughput_ratio": served["throughput_rps"] / plain["throughput_rps"],
             "p50_ratio": plain["p50_s"] / served["p50_s"],
@@ -205,6 +215,9 @@ def serving_leg(args):
             "same_iterations": all(a.iterations == b.iterations
                                    for a, b in zip(served_reqs, plain_reqs)),
             "all_converged": all(r.residual <= 1e-4 for r in served_reqs + plain_reqs)}
+    if small:
+        out["workload"] = "32^3-64^3 cubes, 8-24 views of 64^2-128^2, mask on/off, 1-4 bins"
+    return out
 
 
 def measured_traffic(kernel, args, plan_info):
@@ -387,6 +400,7 @@ def main():
         torch.cuda.empty_cache()
     if args.serve_requests:
         extra["serving"] = serving_leg(args)
+        extra["serving_small"] = serving_leg(args, small=True)
     cpu = None
     if not args.no_cpu:
         y_host = np.ascontiguousarray(y_host)

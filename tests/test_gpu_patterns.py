@@ -91,17 +91,22 @@ def _requests(n, seed, rate=None):
     return SV.make_requests(mix, rng, rate=rate, tol=1e-4, maxiter=300)
 
 
-def test_served_results_match_plain():
-    """Each reconstruction is the same alone (plain) and in the mix (served):
-    within 1e-4 relative L2, both converged to the tolerance."""
+@pytest.mark.parametrize("speculative", [False, True, "auto"])
+def test_served_results_match_plain(speculative):
+    """Each reconstruction is the same alone (plain) and in the mix (served),
+    with residual checks read at once or one step late: within 1e-4 relative
+    L2 (identical in fact, with float64 backprojection sums), same iteration
+    counts, both converged to the tolerance."""
+    print("served vs plain, speculative", speculative, flush=True)
     reqs = _requests(10, seed=2)
     plain = _requests(10, seed=2)
-    server = SV.Server(budget_bytes=1 << 30, max_active=8, check_every=5)
+    server = SV.Server(budget_bytes=1 << 30, max_active=8, check_every=5, speculative=speculative)
     s_sum = server.run(reqs)
     p_sum = SV.serve_plain(plain, check_every=5)
     assert s_sum["requests"] == p_sum["requests"] == 10
     for a, b in zip(reqs, plain):
         assert a.residual <= 1e-4 and b.residual <= 1e-4
+        assert a.iterations == b.iterations, (a.id, a.iterations, b.iterations)
         assert rel_l2(a.x, b.x) <= 1e-4, (a.id, rel_l2(a.x, b.x))
     assert s_sum["p99_s"] >= s_sum["p50_s"] > 0
 

@@ -83,90 +83,132 @@ def make_plan(tile_j=0, blocks=0, schedule=1):
     return p
 
 
+class Launch:
+    """A batched launch prepared once and run any number of times.
+
+    The job calls (and, for a backward batch, the split between brick-plan and
+    atomic jobs) are packed at construction; every ``run`` launches them again
+    on the operands' current contents.  A CG engine step over a fixed set of
+    solves therefore packs its calls once, not once per iteration.
+
+    ops: Siddon operators (same dtype); ins/outs: device tensors shaped like the
+    operator's cube / image stack.  Outputs are accumulated into, as in the
+    one-job calls; with zero_out=True a forward launch overwrites them as if
+    they had been zeroed.  Voxel operands of a forward job are applied to its
+    map before each launch; voxel weights of a backward Thomson job are applied
+    to its output after it.
+    """
+
+    def __init__(self, ops, ins, outs, backward=False, zero_out=False):
+        n = len(ops)
+        self.n = n
+        self.backward = bool(backward)
+        self.zero_out = bool(zero_out)
+        self.dev = ops[0].dev if n else None
+        self.refresh = []    # (operator-layout copy, caller's operand): refreshed per run
+        self.copies = []     # (operator-layout output, caller's output): copied back per run
+        self.post = []       # (operator, temporary, output): backward voxel weights
+        self.prepass = []    # (operator, map, weighted-map buffer): forward voxel operands
+        self.alive = []      # every tensor the packed calls point to
+        calls = (N.TgCall * n)()
+        weighted = set()
+        for k, (P, x, y) in enumerate(zip(ops, ins, outs)):
+            if self.backward:
+                target = y
+                if P.vox_weight is not None:
+                    target = torch.zeros_like(y)
+                    self.post.append((P, target, y))
+                data = self._operand(P._as_stack(x), x)
+                c = P._call(N.TG_BPJ, target, data)
+            else:
+                target = y
+                if P.pix_weight is not None and y.stride() != P.pix_weight.stride():
+                    # pixel weights live in the operator's view-major layout
+                    target = P._as_stack(y)
+                    self.copies.append((target, y))
+                cube = self._operand(P._as_cube(x), x)
+                if P.vox_weight is not None or P.voxmask is not None:
+                    # an operator's weighted-map buffer serves its first job only
+                    wmap = P.vox_buffer(torch.empty_like(P.xin) if id(P) in weighted else None)
+                    weighted.add(id(P))
+                    self.prepass.append((P, cube, wmap))
+                    self.alive.append(wmap)
+                    cube = wmap
+                c = P._call(N.TG_PJ, cube, target, apply_vox=False)
+                if self.zero_out:
+                    c.flags |= N.TG_ZERO_OUT
+            self.alive += [target, P]
+            ctypes.memmove(ctypes.byref(calls[k]), ctypes.byref(c), ctypes.sizeof(N.TgCall))
+        self.calls = calls
+        self.pcalls = self.plans = self.acalls = None
+        if self.backward:
+            # brick-plan jobs in one plan launch, the others in one atomic launch
+            planned = [k for k, P in enumerate(ops) if P.use_plan()]
+            rest = sorted(set(range(n)) - set(planned))
+            if planned:
+                self.pcalls = (N.TgCall * len(planned))()
+                self.plans = (N.TgBpjPlan * len(planned))()
+                for q, k in enumerate(planned):
+                    ctypes.memmove(ctypes.byref(self.pcalls[q]), ctypes.byref(calls[k]),
+                                   ctypes.sizeof(N.TgCall))
+                    p = ops[k].build_plan()
+                    ctypes.memmove(ctypes.byref(self.plans[q]), ctypes.byref(p),
+                                   ctypes.sizeof(N.TgBpjPlan))
+            if rest:
+                self.acalls = (N.TgCall * len(rest))()
+                for q, k in enumerate(rest):
+                    ctypes.memmove(ctypes.byref(self.acalls[q]), ctypes.byref(calls[k]),
+                                   ctypes.sizeof(N.TgCall))
+
+    def _operand(self, t, x):
+        """The operator-layout operand t of the caller's x (a copy when layouts,
+        devices or dtypes differ, then refreshed on every run)."""
+        self.alive.append(t)
+        if not isinstance(x, torch.Tensor) or t.data_ptr() != x.data_ptr():
+            self.refresh.append((t, x))
+        return t
+
+    def run(self, plan=None):
+        if self.n == 0:
+            return
+        for t, x in self.refresh:
+            t.copy_(torch.as_tensor(x).reshape(t.shape))
+        for P, cube, wmap in self.prepass:
+            P.vox_apply(cube, wmap)
+        if not self.zero_out:
+            for tmp, y in self.copies:
+                tmp.copy_(y)
+        for P, tmp, y in self.post:
+            tmp.zero_()
+        s = S.stream_handle()
+        if self.backward:
+            if self.pcalls is not None:
+                m = len(self.pcalls)
+                ring = _ring("plan", self.dev)
+                b = ring.take(int(N.lib().tg_bpj_plan_batch_bytes(m)), self.dev)
+                N.check(N.lib().tg_bpj_plan_batch(self.pcalls, self.plans, m, b.queue.data_ptr(),
+                                                  b.stage.data_ptr(), s))
+                ring.release(b)
+            if self.acalls is not None:
+                _batch_launch(self.acalls, len(self.acalls), plan, s, self.dev)
+        else:
+            _batch_launch(self.calls, self.n, plan, s, self.dev)
+        for tmp, y in self.copies:
+            y.copy_(tmp)
+        for P, tmp, y in self.post:
+            sc = S.make_call(N.TG_BPJ, P.code, P.xout.shape, P.xout.stride(), tmp.shape,
+                             tmp.stride(), P.map_rec, P.views, tmp.data_ptr(), P.xout.data_ptr(),
+                             vox_weight=P.vox_weight)
+            N.check(N.lib().tg_scale_cube(ctypes.byref(sc), s))
+            y.add_(tmp)
+
+
 def run(ops, ins, outs, backward=False, plan=None, zero_out=False):
     """outs[k] += P_k ins[k] (or P_k^T ins[k] when backward) for all k, on the
     current stream: one persistent launch for a forward batch; for a backward
     batch, one launch over the brick plans of the jobs that use one and one
-    persistent atomic launch over the others.
-
-    ops: Siddon operators (same dtype); ins/outs: device tensors shaped like the
-    operator's cube / image stack (outputs are accumulated into, as in the
-    one-job calls; with zero_out=True a forward launch overwrites them as if
-    they had been zeroed).  Voxel weights of a backward Thomson job are applied
-    to its output after the launch.
-    """
-    n = len(ops)
-    if n == 0:
-        return outs
-    calls = (N.TgCall * n)()
-    post = []
-    copies = []
-    alive = []  # temporaries the launches read: referenced until they are enqueued
-    weighted = set()
-    for k, (P, x, y) in enumerate(zip(ops, ins, outs)):
-        if backward:
-            target = y
-            if P.vox_weight is not None:
-                target = torch.zeros_like(y)
-                post.append((P, target, y))
-            data = P._as_stack(x)
-            alive.append(data)
-            c = P._call(N.TG_BPJ, target, data)
-        else:
-            target = y
-            if P.pix_weight is not None and y.stride() != P.pix_weight.stride():
-                # pixel weights live in the operator's view-major layout
-                target = P._as_stack(y)
-                copies.append((target, y))
-            cube = P._as_cube(x)
-            alive.append(cube)
-            vox_out = None
-            if P.vox_weight is not None or P.voxmask is not None:
-                # an operator's weighted-map buffer serves its first job only
-                if id(P) in weighted:
-                    vox_out = torch.empty_like(P.xin)
-                    alive.append(vox_out)
-                weighted.add(id(P))
-            c = P._call(N.TG_PJ, cube, target, vox_out=vox_out)
-            if zero_out:
-                c.flags |= N.TG_ZERO_OUT
-        ctypes.memmove(ctypes.byref(calls[k]), ctypes.byref(c), ctypes.sizeof(N.TgCall))
-    s = S.stream_handle()
-    dev = ops[0].dev
-    if backward:
-        # brick-plan jobs in one plan launch, the others in one atomic launch
-        planned = [k for k, P in enumerate(ops) if P.use_plan()]
-        rest = sorted(set(range(n)) - set(planned))
-        if planned:
-            m = len(planned)
-            pcalls = (N.TgCall * m)()
-            plans = (N.TgBpjPlan * m)()
-            for q, k in enumerate(planned):
-                ctypes.memmove(ctypes.byref(pcalls[q]), ctypes.byref(calls[k]), ctypes.sizeof(N.TgCall))
-                p = ops[k].build_plan()
-                ctypes.memmove(ctypes.byref(plans[q]), ctypes.byref(p), ctypes.sizeof(N.TgBpjPlan))
-            ring = _ring("plan", dev)
-            b = ring.take(int(N.lib().tg_bpj_plan_batch_bytes(m)), dev)
-            N.check(N.lib().tg_bpj_plan_batch(pcalls, plans, m, b.queue.data_ptr(),
-                                              b.stage.data_ptr(), s))
-            ring.release(b)
-        if rest:
-            m = len(rest)
-            acalls = (N.TgCall * m)()
-            for q, k in enumerate(rest):
-                ctypes.memmove(ctypes.byref(acalls[q]), ctypes.byref(calls[k]), ctypes.sizeof(N.TgCall))
-            _batch_launch(acalls, m, plan, s, dev)
-    else:
-        _batch_launch(calls, n, plan, s, dev)
-    del alive
-    for tmp, y in copies:
-        y.copy_(tmp)
-    for P, tmp, y in post:
-        sc = S.make_call(N.TG_BPJ, P.code, P.xout.shape, P.xout.stride(), tmp.shape, tmp.stride(),
-                         P.map_rec, P.views, tmp.data_ptr(), P.xout.data_ptr(),
-                         vox_weight=P.vox_weight)
-        N.check(N.lib().tg_scale_cube(ctypes.byref(sc), S.stream_handle()))
-        y.add_(tmp)
+    persistent atomic launch over the others.  See ``Launch`` for the operands."""
+    Launch(ops, ins, outs, backward, zero_out).run(plan)
     return outs
 
 

@@ -116,9 +116,12 @@ class SegTable:
         self.nblocks = block
         raw = ctypes.string_at(ctypes.addressof(segs), ctypes.sizeof(segs))
         host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
-        self.dev_table = host.to(dev)
+        # from pinned memory the upload is stream-ordered and does not wait for
+        # the work already queued
+        self.dev_table = host.pin_memory().to(dev, non_blocking=True)
         self.partials = torch.empty(max(block, 1), dtype=torch.float64, device=dev)
         self.code = solves[0].P.code
+        self.launches = None  # the batched forward and backward of a step, packed once
 
     def op(self, op):
         N.check(N.lib().tg_cg_batch(self.code, op, self.dev_table.data_ptr(), self.nseg,
@@ -143,16 +146,22 @@ def iterate(solves, table, tuner=None):
         _forward_one(solves[0])
         _backward_one(solves[0])
     else:
-        ops = [s.P for s in solves]
-        ins = [s.view(s.ps) for s in solves]
-        outs = [s.out for s in solves]
-        plan_f = tuner.plan_for(ops, ins, outs, False, zero_out=True) if tuner else None
-        B.run(ops, ins, outs, backward=False, plan=plan_f, zero_out=True)
-        # contended geometries backproject through their brick plans in one
-        # launch, the others in one persistent atomic launch
-        qs = [s.view(s.qs) for s in solves]
-        plan_b = tuner.plan_for(ops, outs, qs, True) if tuner else None
-        B.run(ops, outs, qs, backward=True, plan=plan_b)
+        if table.launches is None:
+            # the operands of a step are the same buffers every iteration: pack
+            # the batched calls (and pick the tuned plans) once per set of solves
+            ops = [s.P for s in solves]
+            ins = [s.view(s.ps) for s in solves]
+            outs = [s.out for s in solves]
+            qs = [s.view(s.qs) for s in solves]
+            plan_f = tuner.plan_for(ops, ins, outs, False, zero_out=True) if tuner else None
+            plan_b = tuner.plan_for(ops, outs, qs, True) if tuner else None
+            table.launches = (B.Launch(ops, ins, outs, backward=False, zero_out=True), plan_f,
+                              B.Launch(ops, outs, qs, backward=True), plan_b)
+        fwd, plan_f, bwd, plan_b = table.launches
+        fwd.run(plan_f)
+        # geometries with a brick plan backproject in one plan launch, the
+        # others in one persistent atomic launch
+        bwd.run(plan_b)
     table.op(N.TG_CG_OP_REGULARIZE)
     table.op(N.TG_CG_OP_DOT_PQ)
     table.op(N.TG_CG_OP_UPDATE)

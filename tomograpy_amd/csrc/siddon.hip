@@ -310,6 +310,46 @@ int finish_acc(const KArgs<T> &a, hipStream_t s) {
 template int finish_acc<float>(const KArgs<float> &, hipStream_t);
 template int finish_acc<double>(const KArgs<double> &, hipStream_t);
 
+// k_acc_finish over the jobs of a device table: blockIdx.y is the job.
+template <typename T>
+__global__ __launch_bounds__(256) void k_acc_finish_table(const char *__restrict__ table,
+                                                          size_t stride) {
+  const KArgs<T> &a = *reinterpret_cast<const KArgs<T> *>(table + (size_t)blockIdx.y * stride);
+  if (!a.acc) return;
+  const int64_t total = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.nt;
+  for (int64_t k = blockIdx.x * 256ll + threadIdx.x; k < total; k += (int64_t)gridDim.x * 256) {
+    const double v = a.acc[k];
+    if (v != 0.0) {
+      a.cube[k] += (T)v;
+      a.acc[k] = 0.0;
+    }
+  }
+}
+
+template <typename T>
+int finish_acc_table(const void *dev_table, size_t stride, const KArgs<T> *const *host_args,
+                     int njobs, hipStream_t s) {
+  int64_t most = 0;
+  for (int k = 0; k < njobs; k++) {
+    const KArgs<T> &a = *host_args[k];
+    if (!a.acc) continue;
+    const int64_t total = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.nt;
+    most = total > most ? total : most;
+  }
+  if (most == 0) return TG_OK;
+  if (njobs > 65535) return set_error(TG_EINVAL, "accumulator finish of %d jobs", njobs);
+  const int64_t want = (most + 255) / 256;
+  const uint32_t bx = (uint32_t)(want < 1024 ? want : 1024);
+  hipLaunchKernelGGL((k_acc_finish_table<T>), dim3(bx, (uint32_t)njobs), dim3(256), 0, s,
+                     (const char *)dev_table, stride);
+  return hip_check(hipGetLastError(), "accumulator finish launch");
+}
+
+template int finish_acc_table<float>(const void *, size_t, const KArgs<float> *const *, int,
+                                     hipStream_t);
+template int finish_acc_table<double>(const void *, size_t, const KArgs<double> *const *, int,
+                                      hipStream_t);
+
 template <typename T>
 int run_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, hipStream_t s) {
   KArgs<T> a;

@@ -213,9 +213,13 @@ int run_batch(const tg_call *calls, int njobs, const tg_batch_plan *plan, void *
       launch_group<T, false>(v, b, s, jobs + gfirst[v], gcount[v], gtiles[v], hdr + kHeadWord + v);
   }
   if ((rc = hip_check(hipGetLastError(), "batch kernel launch"))) return rc;
-  if (bwd)
-    for (int k = 0; k < njobs; k++)
-      if ((rc = finish_acc<T>(args[k], s))) return rc;
+  if (bwd) {
+    // the jobs' float64 accumulators, finished in one launch over the device table
+    static_assert(offsetof(JobDesc<T>, a) == 0, "finish_acc_table reads KArgs at entry start");
+    std::vector<const KArgs<T> *> hargs(njobs);
+    for (int k = 0; k < njobs; k++) hargs[k] = &args[k];
+    if ((rc = finish_acc_table<T>(jobs, sizeof(JobDesc<T>), hargs.data(), njobs, s))) return rc;
+  }
   return TG_OK;
 }
 

@@ -17,6 +17,13 @@ int hip_check(hipError_t e, const char *what);
 template <typename T>
 int finish_acc(const KArgs<T> &a, hipStream_t s);
 
+// finish_acc for every job of a batch in one launch: the jobs' KArgs<T> sit at
+// the start of `stride`-byte entries of the device table `dev_table`; host_args
+// are the same jobs' arguments on the host (for the launch size).
+template <typename T>
+int finish_acc_table(const void *dev_table, size_t stride, const KArgs<T> *const *host_args,
+                     int njobs, hipStream_t s);
+
 // Deterministic backprojection (siddon_det.hip).
 template <typename T>
 int run_deterministic(const tg_call *c, const KArgs<T> &a, hipStream_t s);

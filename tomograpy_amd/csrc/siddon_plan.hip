@@ -754,9 +754,12 @@ int plan_bpj_batch(const tg_call *calls, const tg_bpj_plan *plans, int njobs, vo
                        (const PlanJob<T> *)((const char *)queue + kPlanQueueHeader), njobs);
     if ((rc = hip_check(hipGetLastError(), "plan batch launch"))) return rc;
   }
-  for (int k = 0; k < njobs; k++)
-    if ((rc = finish_acc<T>(table[k].a, s))) return rc;
-  return TG_OK;
+  // the jobs' float64 accumulators, finished in one launch over the device table
+  static_assert(offsetof(PlanJob<T>, a) == 0, "finish_acc_table reads KArgs at entry start");
+  std::vector<const KArgs<T> *> hargs(njobs);
+  for (int k = 0; k < njobs; k++) hargs[k] = &table[k].a;
+  return finish_acc_table<T>((const char *)queue + kPlanQueueHeader, sizeof(PlanJob<T>),
+                             hargs.data(), njobs, s);
 }
 
 }  // namespace

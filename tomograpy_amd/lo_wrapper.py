@@ -278,14 +278,19 @@ class Siddon:
         the float64 backprojection accumulator needs)."""
         return torch.zeros_like(self.xin)
 
+    def vox_buffer(self, out=None):
+        """`out`, or the operator's own buffer for the voxel-weighted map."""
+        if out is not None:
+            return out
+        if self._xw is None:
+            self._xw = torch.empty_like(self.xin)
+        return self._xw
+
     def vox_apply(self, cube, out=None):
         """out = f * cube with masked voxels zeroed (models.py:223-235 and
         155-165), launched on the current stream; `out` defaults to the
         operator's buffer.  `cube` has the operator's input shape."""
-        if out is None:
-            if self._xw is None:
-                self._xw = torch.empty_like(self.xin)
-            out = self._xw
+        out = self.vox_buffer(out)
         time = cube.dim() == 4
         c = S.make_call(N.TG_PJT if time else N.TG_PJ, self.code, self.xout.shape,
                         self.xout.stride(), out.shape, out.stride(), self.map_rec, self.views,

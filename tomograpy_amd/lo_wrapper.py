@@ -14,6 +14,7 @@ adjacent in memory; the cube is C-contiguous (N0, N1, N2), and a 4-D map is
 stored bin-major (T, N0, N1, N2) and presented as (N0, N1, N2, T).
 """
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -167,10 +168,10 @@ class Siddon:
         adjacent pixels add into the same LDS cells and spreading them pays
         (55^3 seen in 512^2 pixels, ratio 8.7: 2.36 -> 1.19 ms; 128^3, ratio
         3.7: 2.52 -> 2.05 ms with 6 bits; 256^3, ratio 1.9: 6 bits cost 8%)."""
-        rec = view_records(self.data_header)
-        dist = np.linalg.norm(rec[:, 15:18], axis=1)
-        pix = float(np.median(dist * np.abs(rec[:, 3])))
-        vox = float(min(abs(v) for v in self.map_rec[0:3]))
+        foot = sorted(math.sqrt(float(h["M1"]) ** 2 + float(h["M2"]) ** 2 + float(h["M3"]) ** 2)
+                      * abs(float(h["CDELT2"])) for h in self.data_header)
+        pix = foot[len(foot) // 2] if foot else 0.0
+        vox = float(min(abs(float(v)) for v in self.map_rec[0:3]))
         ratio = vox / pix if pix > 0 else 0.0
         return 0 if ratio < 3.0 else (6 if ratio < 4.5 else 10)
 

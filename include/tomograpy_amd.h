@@ -173,7 +173,8 @@ int tg_vox_apply(const tg_call *c, const void *src, const int64_t src_stride[4],
  * continuous traversal; only the summation order into a voxel differs (as
  * in the atomic mode). */
 typedef struct tg_bpj_plan {
-  int32_t brick[3];      /* brick edge per axis, powers of two; 0,0,0 = default (128 KiB) */
+  int32_t brick[3];      /* brick edge per axis, powers of two; 0,0,0 = default: 32 x 16 x 32
+                            (128 KiB), or 16 x 16 x 32 for cubes of edge <= 96 */
   int32_t chunk_records; /* records per workgroup chunk; 0 = chosen by tg_bpj_plan_count
                             (about 1024 chunks of 2K..64K records) and written back */
   int32_t nb_axis[3];    /* bricks per axis (set by tg_bpj_plan_init) */

@@ -169,6 +169,7 @@ def test_batch_job_table_layout(dtype_code, schedule, mixed):
 
 
 @pytest.mark.parametrize("dtype_code,n,brick", [(N.TG_F32, 40, (0, 0, 0)), (N.TG_F64, 70, (0, 0, 0)),
+                                                (N.TG_F32, 130, (0, 0, 0)),
                                                 (N.TG_F32, 33, (8, 16, 32))])
 def test_bpj_plan_init_fields(dtype_code, n, brick):
     """tg_bpj_plan_init (host only) fills the fields where the ctypes mirror expects them."""
@@ -186,7 +187,8 @@ def test_bpj_plan_init_fields(dtype_code, n, brick):
     N.check(N.lib().tg_bpj_plan_init(ctypes.byref(call), ctypes.byref(p)))
     b = list(p.brick)
     if brick == (0, 0, 0):
-        assert b == [32, 16, 32]  # 128 KiB of double accumulators
+        # 128 KiB of double accumulators, 64 KiB for cubes of edge <= 96
+        assert b == ([16, 16, 32] if n <= 96 else [32, 16, 32])
     nb = [(n + b[k] - 1) // b[k] for k in range(3)]
     assert list(p.nb_axis) == nb
     assert p.nbricks == nb[0] * nb[1] * nb[2] * 3

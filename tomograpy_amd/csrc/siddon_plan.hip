@@ -772,8 +772,14 @@ extern "C" {
 int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p) {
   if (!c || !p) return set_error(TG_EINVAL, "null argument");
   if (p->brick[0] == 0 && p->brick[1] == 0 && p->brick[2] == 0) {
-    // default: 32 x 16 x 32 voxels = 128 KiB of double accumulators
-    p->brick[0] = 32;
+    // default: 32 x 16 x 32 voxels = 128 KiB of double accumulators; for cubes
+    // of edge <= 96, 16 x 16 x 32 (64 KiB, two workgroups per CU), whose
+    // per-chunk zeroing and flush cost less against the few records of a
+    // small cube (48^3, 16 x 96^2: 0.025 -> 0.022 ms; 96^3, 32 x 128^2:
+    // 0.105 -> 0.090 ms; 128^3, 64 x 512^2: 2.03 against 2.11 ms)
+    const int most = c ? (c->map.n[0] > c->map.n[1] ? c->map.n[0] : c->map.n[1]) : 0;
+    const int edge = c ? (most > c->map.n[2] ? most : c->map.n[2]) : 0;
+    p->brick[0] = edge <= 96 ? 16 : 32;
     p->brick[1] = 16;
     p->brick[2] = 32;
   }

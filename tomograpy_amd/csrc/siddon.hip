@@ -153,6 +153,37 @@ __global__ __launch_bounds__(256) void k_vox_apply(KArgs<T> a, const T *__restri
   }
 }
 
+// k_vox_apply for a source with the time bins fastest (the (N0, N1, N2, T)
+// C-order array) into a bin-major cube: a workgroup reads 256 consecutive
+// voxels x nt bins in one coalesced sweep, transposes them through LDS, and
+// writes each bin's 256 voxels as one coalesced run.  Voxel operands and the
+// destination are dense (strides n1 n2, n2, 1 and nvox per bin).
+constexpr int kVoxTile = 256;
+constexpr int kVoxTileBins = 16;
+
+template <typename T>
+__global__ __launch_bounds__(kVoxTile) void k_vox_apply_bins(KArgs<T> a, const T *__restrict__ src) {
+  __shared__ T tile[kVoxTileBins][kVoxTile + 1];
+  const int nt = a.nt;
+  const int64_t nvox = (int64_t)a.n[0] * a.n[1] * a.n[2];
+  for (int64_t v0 = (int64_t)blockIdx.x * kVoxTile; v0 < nvox; v0 += (int64_t)gridDim.x * kVoxTile) {
+    const int nv = (int)(nvox - v0 < kVoxTile ? nvox - v0 : kVoxTile);
+    for (int i = threadIdx.x; i < nv * nt; i += kVoxTile) tile[i % nt][i / nt] = src[v0 * nt + i];
+    __syncthreads();
+    if ((int)threadIdx.x < nv) {
+      const int64_t v = v0 + threadIdx.x;
+      const T w = a.voxw ? a.voxw[v] : (T)1;
+      const bool masked = a.voxmask && a.voxmask[v] != 0;
+      for (int b = 0; b < nt; b++) {
+        T x = tile[b][threadIdx.x];
+        if (a.voxw) x = w * x;
+        a.cube[(int64_t)b * nvox + v] = masked ? (T)0 : x;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -458,6 +489,19 @@ int run_vox_apply(const tg_call *c, const void *src, const int64_t src_stride[4]
   if (src == (const void *)c->cube) return set_error(TG_EINVAL, "vox_apply: src and cube must differ");
   const int64_t total = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.nt;
   if (total == 0) return TG_OK;
+  const int64_t n1 = a.n[1], n2 = a.n[2], nt = a.nt, nvox = (int64_t)a.n[0] * n1 * n2;
+  const bool bins_fastest = nt > 1 && nt <= kVoxTileBins && src_stride[3] == 1 &&
+                            src_stride[2] == nt && src_stride[1] == n2 * nt &&
+                            src_stride[0] == n1 * n2 * nt;
+  const bool dense_dst = a.cs[0] == n1 * n2 && a.cs[1] == n2 && a.cs[2] == 1 && a.cs[3] == nvox;
+  const bool dense_vox = (!a.voxw && !a.voxmask) ||
+                         (a.vs[0] == n1 * n2 && a.vs[1] == n2 && a.vs[2] == 1);
+  if (bins_fastest && dense_dst && dense_vox) {
+    const int64_t want = (nvox + kVoxTile - 1) / kVoxTile;
+    const uint32_t blocks = (uint32_t)(want < 65536 ? want : 65536);
+    hipLaunchKernelGGL((k_vox_apply_bins<T>), dim3(blocks), dim3(kVoxTile), 0, s, a, (const T *)src);
+    return hip_check(hipGetLastError(), "vox_apply kernel launch");
+  }
   const uint32_t blocks = (uint32_t)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
   hipLaunchKernelGGL((k_vox_apply<T>), dim3(blocks), dim3(256), 0, s, a, (const T *)src,
                      src_stride[0], src_stride[1], src_stride[2], src_stride[3]);

@@ -78,8 +78,9 @@ def _event_ms(fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def cg_run(P, y, iters):
-    """Times `iters` CG iterations on (P^T P + 1e-2 (I + sum D_k^T D_k)) x = P^T y."""
+def cg_run(P, y, iters, keep=None):
+    """Times `iters` CG iterations on (P^T P + 1e-2 (I + sum D_k^T D_k)) x = P^T y;
+    with a dict `keep`, stores the solution there (host, float64)."""
     import torch
     from tomograpy_amd import cg as CG
     stream = torch.cuda.current_stream()
@@ -87,15 +88,47 @@ def cg_run(P, y, iters):
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    _, info = CG.solve(P, y, lam=1e-2, identity=True, maxiter=iters)
+    x, info = CG.solve(P, y, lam=1e-2, identity=True, maxiter=iters)
     e1.record(stream)
     torch.cuda.synchronize()
+    if keep is not None:
+        keep["x"] = x.detach().double().cpu()
     return {"iterations": info["iterations"], "ms_per_iteration": e0.elapsed_time(e1) / iters,
             "relative_residual": info["residual"]}
 
 
-def fp64_leg(args):
-    """The same workload in float64: the pair and the CG iterations."""
+def _rel(a, b):
+    a = a.double().flatten()
+    b = b.double().flatten()
+    return float((a - b).norm() / max(float(b.norm()), 1e-300))
+
+
+def axis_views(hs):
+    """Views whose observer sits on a coordinate plane up to rounding: a
+    coordinate of M tiny but not zero (LON = pi/2, pi, 3 pi/2 evaluated in
+    floating point).  There the reference's float32 ray set-up gives a few rays
+    steps of about +-212 (C:723-735 with C:804-809); float64 does not."""
+    out = []
+    for t, h in enumerate(hs):
+        m = [float(h["M%d" % (k + 1)]) for k in range(3)]
+        d = float(np.sqrt(sum(v * v for v in m)))
+        if any(0.0 < abs(v) < 1e-9 * d for v in m):
+            out.append(t)
+    return out
+
+
+def view_mask(hs, n1, n2, views):
+    """Pixel mask (True = skip) covering the given views."""
+    m = np.zeros((n1, n2, len(hs)), dtype=bool)
+    m[:, :, views] = True
+    return m
+
+
+def fp64_leg(args, ref32=None):
+    """The same workload in float64: the pair and the CG iterations.  With the
+    float32 run's inputs and results (ref32: x, y, forward, backward, CG
+    solution; host tensors), also the float32 error against float64 on the
+    same inputs."""
     import torch
     from tomograpy_amd import workloads as W
     from tomograpy_amd.lo_wrapper import siddon_lo
@@ -114,14 +147,39 @@ def fp64_leg(args):
 
     out = {"ms_pair": _event_ms(pair, 5), "siddon_steps_per_projection": P.count_steps()}
     out.update(cg_run(P, y, args.cg_iters))
-    return out
+    if ref32 is None:
+        return out, None
+    # the float32 run's inputs, in float64
+    x32 = ref32["x"].cuda()
+    y32 = ref32["y"].cuda()
+    ok = [t for t in range(len(hs)) if t not in set(ref32["axis_views"])]
+    f64 = P.matvec(x32).cpu()
+    err = {"axis_views": ref32["axis_views"],
+           "forward_rel_l2": _rel(f64, ref32["forward"]),
+           "forward_rel_l2_other_views": _rel(f64[:, :, ok], ref32["forward"][:, :, ok]),
+           "backward_rel_l2": _rel(P.rmatvec(y32).cpu(), ref32["backward"]),
+           "backward_rel_l2_other_views": _rel(P.rmatvec(ref32["y_other"].cuda()).cpu(),
+                                               ref32["backward_other"])}
+    keep = {}
+    cg_run(P, y32, args.cg_iters, keep)
+    err["cg_solution_rel_l2"] = _rel(ref32["cg_x"], keep["x"])
+    del P
+    torch.cuda.empty_cache()
+    Pm = siddon_lo(hs, ch, dtype=np.float64,
+                   mask=view_mask(hs, args.pix, args.pix, ref32["axis_views"]))
+    keep = {}
+    cg_run(Pm, y32, args.cg_iters, keep)
+    err["cg_solution_rel_l2_other_views"] = _rel(ref32["cg_x_other"], keep["x"])
+    return out, err
 
 
 def grad_leg():
     """Differentiation workload: 128^3 x 8 time bins, 64 views of 256^2, voxel
     mask r < 1.05; one evaluation of ||P x - y||^2 + 1e-2 ||grad x||^2 +
     1e-2 ||dx/dt||^2 and its gradient (forward through the projector, backward
-    through the hand-written backprojector)."""
+    through the hand-written backprojector).  Also the float32 gradient's error
+    against the float64 one at the same point and data."""
+    import torch
     from tomograpy_amd import workloads as W
     P, y, x0 = W.config3_problem(np.float32)
     loss = W.config3_loss(P, y)
@@ -132,8 +190,38 @@ def grad_leg():
         loss(x).backward()
 
     ms = _event_ms(step, 5)
+    steps = P.count_steps()
+    _, hs, _, _, _ = W.config3_geometry(np.float32)
+    keepv = torch.ones(len(hs), dtype=torch.float64)
+    keepv[axis_views(hs)] = 0.0
+
+    def grads(P, y, x0, dt):
+        """The loss gradient at x0, and the same with the data term of the
+        axis views left out."""
+        out = []
+        for w in (None, keepv.to(P.dev, dt)):
+            xg = x0.clone().requires_grad_(True)
+            if w is None:
+                L = W.config3_loss(P, y)(xg)
+            else:
+                from tomograpy_amd.autograd import project, smoothness
+                r = (project(xg, P) - y) * w
+                L = (r * r).sum() + smoothness(xg, (1e-2, 1e-2, 1e-2, 1e-2))
+            L.backward()
+            out.append(xg.grad.detach().double().cpu())
+        return out
+
+    g32, g32o = grads(P, y, x0, torch.float32)
+    y32, x32 = y.double(), x0.double()
+    del P, loss, x
+    torch.cuda.empty_cache()
+    P64, _, _ = W.config3_problem(np.float64)
+    g64, g64o = grads(P64, y32, x32, torch.float64)
     return {"workload": "128^3 x 8 bins, 64 views of 256^2, voxel mask r<1.05, fp32",
-            "ms_per_loss_and_gradient": ms, "siddon_steps_per_projection": P.count_steps()}
+            "ms_per_loss_and_gradient": ms, "siddon_steps_per_projection": steps,
+            "gradient_rel_l2_vs_fp64": _rel(g32, g64),
+            "gradient_rel_l2_vs_fp64_other_views": _rel(g32o, g64o),
+            "axis_views": axis_views(hs)}
 
 
 def batch_leg():
@@ -381,8 +469,27 @@ def main():
     achieved = B_dom / t_dom / 1e9
 
     extra = {}
+    # the float32 results whose error against float64 the fp64 leg states
+    ref32 = None
+    if dtype == np.float32 and args.cg_iters:
+        av = axis_views(hs)
+        y_other = y.clone()
+        y_other[:, :, av] = 0
+        ref32 = {"x": x_true.double().cpu(), "y": y.double().cpu(), "axis_views": av,
+                 "forward": P.xout.double().cpu(), "backward": P.xin.double().cpu(),
+                 "y_other": y_other.double().cpu(),
+                 "backward_other": P.rmatvec(y_other).double().cpu()}
+        del y_other
     if args.cg_iters:
-        extra["cg"] = {"fp32": cg_run(P, y, args.cg_iters)}
+        keep = {} if ref32 is not None else None
+        extra["cg"] = {"fp32": cg_run(P, y, args.cg_iters, keep)}
+        if keep:
+            ref32["cg_x"] = keep["x"]
+            # the same solve with the axis views' pixels masked out
+            Pm = siddon_lo(hs, ch, dtype=dtype, mask=view_mask(hs, args.pix, args.pix, av))
+            cg_run(Pm, y, args.cg_iters, keep)
+            ref32["cg_x_other"] = keep["x"]
+            del Pm
 
     y_host = y.cpu().numpy() if not args.no_cpu else None
     if args.serve_requests or not args.no_grad or not args.no_batch or args.cg_iters:
@@ -390,7 +497,9 @@ def main():
         del P, x, x_true, y, noise
         torch.cuda.empty_cache()
     if args.cg_iters and dtype == np.float32:
-        extra["cg"]["fp64"] = fp64_leg(args)
+        extra["cg"]["fp64"], err64 = fp64_leg(args, ref32)
+        extra["error_vs_fp64"] = err64
+        ref32 = None
         torch.cuda.empty_cache()
     if not args.no_grad:
         extra["differentiation"] = grad_leg()

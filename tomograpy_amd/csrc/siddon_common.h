@@ -256,7 +256,15 @@ __device__ __forceinline__ T step(Ray<T> &r, bool &sx, bool &sy, bool &sz) {
 constexpr int kNoStep = 1 << 22;
 
 __device__ __forceinline__ float tmin(float a, float b) { return fminf(a, b); }
-__device__ __forceinline__ double tmin(double a, double b) { return fmin(a, b); }
+// The smaller of two doubles by one v_min_f64.  fmin would first quiet each
+// operand (a v_max_f64 x, x, x apiece) to give a signalling NaN fmin's
+// semantics; the traversal distances are never NaN (ray_state_ok rejects such
+// rays before their walk), and for all other inputs the results are the same.
+__device__ __forceinline__ double tmin(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
 
 // A value no larger than x on any active lane of the wavefront, found from the
 // candidate c: while some active lane holds less, c moves to the value of the

@@ -1,7 +1,6 @@
 """CPU pins of the model-level pieces: Thomson coefficients, impact parameter,
 masks and the regularisation stencil reference used by the GPU tests."""
 import numpy as np
-import pytest
 
 import tomograpy_amd as T
 from tomograpy_amd import models as Mo

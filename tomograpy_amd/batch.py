@@ -10,7 +10,6 @@ the fastest per batch signature.
 """
 import ctypes
 
-import numpy as np
 import torch
 
 from . import _native as N

@@ -5,10 +5,11 @@ The server admits a waiting request as soon as its HBM footprint (CG
 vectors, images and, for a large cube, its brick plan) fits the budget and a
 slot is free, keeps its cube, images and CG state on the device, and
 advances every admitted request by one CG iteration per engine step: one
-batched projector launch over all of them (``tg_batch_run``), the brick-plan
-backprojection of each large cube, one batched backprojector launch over the
-others, and the segmented CG vector kernels (``tg_cg_batch``).  A request leaves when its relative residual reaches its
-tolerance (checked every ``check_every`` of its iterations) or at maxiter;
+batched projector launch over all of them (``tg_batch_run``), one brick-plan
+backprojection launch over the requests with a plan (``tg_bpj_plan_batch``),
+one batched backprojector launch over the others, and the segmented CG vector
+kernels (``tg_cg_batch``).  A request leaves when its relative residual reaches
+its tolerance (checked every ``check_every`` of its iterations) or at maxiter;
 requests that arrive meanwhile join at the next engine step.  Image stacks
 come from pinned host memory and solutions go back to pinned buffers with
 stream-ordered copies, so admissions and completions do not stall the GPU work
@@ -26,7 +27,6 @@ import torch
 
 from . import _native as N
 from . import cg as CG
-from .batch import Autotuner
 from .lo_wrapper import Siddon, Siddon4d
 
 
@@ -197,7 +197,7 @@ class Server:
         step runs (a request ended by it has taken one more step, discarded: its
         solution is the one snapshotted at the check), instead of waiting for the
         check before the next step.  That keeps the GPU fed when steps are short
-        (small requests: 1.80x -> 2.13x plain throughput on 64 requests of
+        (small requests: 2.14x -> 2.33x plain throughput on 64 requests of
         32^3-64^3) and costs the discarded steps when they are long (0.98x
         instead of 1.00x on the 32^3-256^3 mix).  "auto" speculates while the
         active set's work per step is under SPECULATE_WORK."""

@@ -288,9 +288,17 @@ def serving_leg(args, small=False):
     served_reqs = requests()
     torch.cuda.synchronize()
     print("serving: %d requests made" % len(plain_reqs), file=sys.stderr, flush=True)
+    tag = "serving_small" if small else "serving"
+    # marker ranges (roctx) that a trace of this run can cut its phases by
+    torch.cuda.nvtx.range_push(tag + " plain")
     plain = SV.serve_plain(plain_reqs)
+    torch.cuda.synchronize()
+    torch.cuda.nvtx.range_pop()
     print("serving: plain done in %.1f s" % plain["makespan_s"], file=sys.stderr, flush=True)
+    torch.cuda.nvtx.range_push(tag + " served")
     served = SV.Server(budget_bytes=128 << 30, max_active=slots).run(served_reqs)
+    torch.cuda.synchronize()
+    torch.cuda.nvtx.range_pop()
     print("serving: served done in %.1f s" % served["makespan_s"], file=sys.stderr, flush=True)
     err = max(float(np.linalg.norm(a.x - b.x) / max(np.linalg.norm(b.x), 1e-300))
               for a, b in zip(served_reqs, plain_reqs))

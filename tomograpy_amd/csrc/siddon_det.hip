@@ -7,9 +7,13 @@
 //   1. count   contributions per voxel (atomic counters),
 //   2. scan    counts -> segment offsets (exclusive, 64-bit),
 //   3. fill    each contribution d * w with its ray key (t, i, j) into its
-//              voxel's segment (slot order is arbitrary),
-//   4. reduce  one lane per voxel sorts its segment by ray key and adds the
-//              values into the voxel in that order.
+//              voxel's segment, one launch per view in view order; a voxel's
+//              slots are handed out upwards, so its segment holds the views
+//              in order and only the few rays of one view that cross the
+//              voxel are in atomic order,
+//   4. reduce  one lane per voxel insertion-sorts its segment by ray key
+//              (linear work: only those per-view groups are out of order)
+//              and adds the values into the voxel in that order.
 // Chunks run in view order, so the per-voxel order across chunks is the
 // reference's too.  The chunk size adapts to the workspace.
 #include <hip/hip_runtime.h>
@@ -29,7 +33,7 @@ constexpr int kScanTile = kScanBlock * kScanItems;
 template <typename T, bool SUN, bool VOX, bool FILL>
 __global__ __launch_bounds__(256) void k_det_trace(KArgs<T> a, uint32_t *counts,
                                                    const unsigned long long *offsets,
-                                                   uint32_t *keys, T *vals) {
+                                                   uint32_t *keys, T *vals, int key_t0) {
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   int i, j, t;
   if (!lane_pixel(a, tile, i, j, t)) return;
@@ -58,7 +62,7 @@ __global__ __launch_bounds__(256) void k_det_trace(KArgs<T> a, uint32_t *counts,
     sm1 = r.up[1] * a.vs[1];
     sm2 = r.up[2] * a.vs[2];
   }
-  const uint32_t key = ((uint32_t)(t - a.t0) * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 +
+  const uint32_t key = ((uint32_t)(t - key_t0) * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 +
                        (uint32_t)j;
   int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
   while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
@@ -75,10 +79,18 @@ __global__ __launch_bounds__(256) void k_det_trace(KArgs<T> a, uint32_t *counts,
     }
     if (keep) {
       if (FILL) {
-        const uint32_t slot = atomicSub(counts + lin, 1u) - 1u;
-        const unsigned long long pos = offsets[lin] + slot;
-        keys[pos] = key;
-        vals[pos] = d * w;
+        // the counter runs down from the voxel's count: the first taker gets
+        // the segment's first slot
+        const uint32_t left = atomicSub(counts + lin, 1u);
+        if (left == 0u) {
+          // the fill met more contributions than the count pass: never write
+          // outside the voxel's segment
+          flag_status(a.status, TG_EGEOMETRY);
+        } else {
+          const unsigned long long pos = offsets[lin + 1] - left;
+          keys[pos] = key;
+          vals[pos] = d * w;
+        }
       } else {
         atomicAdd(counts + lin, 1u);
       }
@@ -208,21 +220,22 @@ DetLayout det_layout(const tg_call *c) {
 
 template <typename T, bool FILL>
 void launch_det_trace(const KArgs<T> &a, uint32_t blocks, hipStream_t s, bool sun, bool vox,
-                      uint32_t *counts, const unsigned long long *off, uint32_t *keys, T *vals) {
+                      uint32_t *counts, const unsigned long long *off, uint32_t *keys, T *vals,
+                      int key_t0) {
   if (sun) {
     if (vox)
       hipLaunchKernelGGL((k_det_trace<T, true, true, FILL>), dim3(blocks), dim3(256), 0, s, a,
-                         counts, off, keys, vals);
+                         counts, off, keys, vals, key_t0);
     else
       hipLaunchKernelGGL((k_det_trace<T, true, false, FILL>), dim3(blocks), dim3(256), 0, s, a,
-                         counts, off, keys, vals);
+                         counts, off, keys, vals, key_t0);
   } else {
     if (vox)
       hipLaunchKernelGGL((k_det_trace<T, false, true, FILL>), dim3(blocks), dim3(256), 0, s, a,
-                         counts, off, keys, vals);
+                         counts, off, keys, vals, key_t0);
     else
       hipLaunchKernelGGL((k_det_trace<T, false, false, FILL>), dim3(blocks), dim3(256), 0, s, a,
-                         counts, off, keys, vals);
+                         counts, off, keys, vals, key_t0);
   }
 }
 
@@ -260,7 +273,7 @@ int run_deterministic(const tg_call *c, const KArgs<T> &a0, hipStream_t s) {
     a.t1 = t + k;
     const uint32_t blocks = (uint32_t)k * (uint32_t)a.tiles_i * (uint32_t)a.tiles_j;
     hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, counts, L.extent);
-    launch_det_trace<T, false>(a, blocks, s, sun, vox, counts, nullptr, nullptr, nullptr);
+    launch_det_trace<T, false>(a, blocks, s, sun, vox, counts, nullptr, nullptr, nullptr, t);
     hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)L.ntiles), dim3(kScanBlock), 0, s, counts,
                        L.extent, tiles);
     hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kScanBlock), 0, s, tiles, L.ntiles);
@@ -282,7 +295,16 @@ int run_deterministic(const tg_call *c, const KArgs<T> &a0, hipStream_t s) {
       continue;
     }
     if (total > 0) {
-      launch_det_trace<T, true>(a, blocks, s, sun, vox, counts, off, keys, vals);
+      // one launch per view, in view order (stream order), so each voxel's
+      // segment receives the views in order
+      const uint32_t view_blocks = (uint32_t)a.tiles_i * (uint32_t)a.tiles_j;
+      for (int tv = t; tv < t + k; tv++) {
+        KArgs<T> av = a;
+        av.t0 = tv;
+        av.t1 = tv + 1;
+        launch_det_trace<T, true>(av, view_blocks, s, sun, vox, counts, off,
+                                  keys, vals, t);
+      }
       const uint32_t red_blocks =
           (uint32_t)((L.extent + 255) / 256 < 65536 ? (L.extent + 255) / 256 : 65536);
       hipLaunchKernelGGL((k_det_reduce<T>), dim3(red_blocks), dim3(256), 0, s, L.extent, off, keys,

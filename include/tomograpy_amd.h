@@ -146,8 +146,8 @@ int tg_intersection_parameters(const tg_call *c, int32_t t0, int32_t t1, const v
  * uint64, may be NULL).  No reference counterpart: used for throughput units. */
 int tg_count_steps(const tg_call *c, uint32_t *per_ray, unsigned long long *total, void *stream);
 
-/* Minimum workspace for TG_DETERMINISTIC backprojection of *c.  Larger
- * workspaces let more views be processed per pass. */
+/* Minimum workspace for TG_DETERMINISTIC backprojection of *c: one pixel row
+ * of rays per pass.  Larger workspaces take more rows (views) per pass. */
 size_t tg_det_workspace_min_bytes(const tg_call *c);
 
 /* Elementwise x[k] *= w[k] over a strided 3-D (or 4-D with nt bins) cube: the

@@ -87,26 +87,36 @@ def test_backward_c1_deterministic_bit_exact(oracle, dtype, obstacle):
     assert np.array_equal(np.asarray(out), np.asarray(ref))
 
 
-def test_deterministic_small_workspace_chunks_views(oracle):
-    """A workspace that holds only a few views' records still gives the exact order."""
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_deterministic_small_workspace_chunks_views(oracle, dtype):
+    """Workspaces of one pixel row up to a few views still give the exact order.
+
+    The rays are taken in chunks of pixel rows, so small workspaces start and
+    end chunks inside a view."""
     import ctypes
     from tomograpy_amd import _native as N, siddon as S
-    ch, x, data = C.small_case(n=16, npix=32, nv=9, dtype=np.float32)
+    print("deterministic small workspace", np.dtype(dtype).name, flush=True)
+    ch, x, data = C.small_case(n=16, npix=32, nv=9, dtype=dtype)
     rng = np.random.default_rng(8)
-    y = T.InfoArray(rng.normal(size=data.shape).astype(np.float32), header=data.header)
-    ref = np.zeros(x.shape, np.float32)
-    oracle.siddon(oracle.BPJ, view_records(y.header), map_record(ch), ref, y, nthread=1)
+    y = T.InfoArray(rng.normal(size=data.shape).astype(dtype), header=data.header)
+    tgd = N.TG_F64 if dtype == np.float64 else N.TG_F32
     yd = torch.from_numpy(np.asarray(y)).cuda()
-    out = torch.zeros(x.shape, dtype=torch.float32, device="cuda")
-    views = S.views_tensor(view_records(y.header), N.TG_F32, 32, 32, out.device)
-    call = S.make_call(N.TG_BPJ, N.TG_F32, yd.shape, yd.stride(), out.shape, out.stride(),
-                       map_record(ch), views, out.data_ptr(), yd.data_ptr(), flags=N.TG_DETERMINISTIC)
-    need = N.lib().tg_det_workspace_min_bytes(ctypes.byref(call))
-    ws = torch.empty(int(need), dtype=torch.uint8, device="cuda")
-    call.workspace = ws.data_ptr()
-    call.workspace_bytes = int(need)
-    N.check(N.lib().tg_conic_full_projector(ctypes.byref(call), S.stream_handle()))
-    assert np.array_equal(out.cpu().numpy(), ref)
+    for sun in (False, True):
+        ref = np.zeros(x.shape, dtype)
+        oracle.siddon(oracle.BPJ, view_records(y.header), map_record(ch), ref, y, sun=sun, nthread=1)
+        out = torch.zeros(x.shape, dtype=yd.dtype, device="cuda")
+        views = S.views_tensor(view_records(y.header), tgd, 32, 32, out.device)
+        flags = N.TG_DETERMINISTIC | (N.TG_OBSTACLE_SUN if sun else 0)
+        call = S.make_call(N.TG_BPJ, tgd, yd.shape, yd.stride(), out.shape, out.stride(),
+                           map_record(ch), views, out.data_ptr(), yd.data_ptr(), flags=flags)
+        need = int(N.lib().tg_det_workspace_min_bytes(ctypes.byref(call)))
+        for size in (need, 3 * need, 40 * need):
+            out.zero_()
+            ws = torch.empty(size, dtype=torch.uint8, device="cuda")
+            call.workspace = ws.data_ptr()
+            call.workspace_bytes = size
+            N.check(N.lib().tg_conic_full_projector(ctypes.byref(call), S.stream_handle()))
+            assert np.array_equal(out.cpu().numpy(), ref), (sun, size)
     # too small a workspace is an error, not a wrong answer
     call.workspace_bytes = 1024
     assert N.lib().tg_conic_full_projector(ctypes.byref(call), S.stream_handle()) == N.TG_EWORKSPACE

@@ -60,6 +60,8 @@ def parse():
                     help="Poisson arrival rate in requests/s (0 = all requests at once)")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
+    ap.add_argument("--no-shard", action="store_true",
+                    help="skip the leg that times one GPU's share of a view-sharded run")
     return ap.parse_args()
 
 
@@ -316,6 +318,40 @@ def serving_leg(args, small=False):
     return out
 
 
+def shard_leg(args, nshards=8, reps=10):
+    """The pair on one GPU's share of a view-sharded run: view k on shard k mod
+    nshards, every shard at the full cube and image shape, each timed alone."""
+    import torch
+    from tomograpy_amd import workloads as W
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    dtype = np.dtype(args.dtype)
+    ch, hs, rng = W.config2_geometry(dtype, n=args.n, n_views=args.views, n_pix=args.pix)
+    x = torch.from_numpy(rng.uniform(size=(args.n,) * 3).astype(dtype)).cuda()
+    ms, steps = [], []
+    for s in range(nshards):
+        hs_s = hs[s::nshards]
+        P = siddon_lo(hs_s, ch, dtype=dtype, tile_j=args.tile_j)
+        y = torch.from_numpy(rng.normal(size=(args.pix, args.pix, len(hs_s))).astype(dtype)).cuda()
+        if P.use_plan():
+            P.build_plan()
+
+        def pair():
+            P.xout.zero_()
+            P.project_into(x, P.xout)
+            P.xin.zero_()
+            P.backproject_into(y, P.xin)
+
+        ms.append(_event_ms(pair, reps))
+        steps.append(int(P.count_steps()))
+        del P, y
+        torch.cuda.empty_cache()
+    k = int(np.argmax(ms))
+    return {"shards": nshards, "views_per_shard": len(hs[::nshards]),
+            "assignment": "view k on shard k mod %d" % nshards,
+            "ms_pair": [round(m, 4) for m in ms], "ms_pair_max": ms[k],
+            "siddon_steps_per_s_slowest_shard": 2 * steps[k] / (ms[k] * 1e-3)}
+
+
 def measured_traffic(kernel, args, plan_info):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary, when this run is the workload that summary measured
@@ -500,7 +536,8 @@ def main():
             del Pm
 
     y_host = y.cpu().numpy() if not args.no_cpu else None
-    if args.serve_requests or not args.no_grad or not args.no_batch or args.cg_iters:
+    if args.serve_requests or not args.no_grad or not args.no_batch or args.cg_iters \
+            or not args.no_shard:
         # release the pair's operator (its brick plan is 9.4 GB at 256^3) first
         del P, x, x_true, y, noise
         torch.cuda.empty_cache()
@@ -514,6 +551,9 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_batch:
         extra["batch"] = batch_leg()
+        torch.cuda.empty_cache()
+    if not args.no_shard:
+        extra["shard_of_8"] = shard_leg(args)
         torch.cuda.empty_cache()
     if args.serve_requests:
         extra["serving"] = serving_leg(args)

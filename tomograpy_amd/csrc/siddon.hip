@@ -294,10 +294,6 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a, int tile_j = 0) {
 template int make_args<float>(const tg_call *, int, int, KArgs<float> &, int);
 template int make_args<double>(const tg_call *, int, int, KArgs<double> &, int);
 
-template <typename T>
-uint32_t grid_blocks(const KArgs<T> &a) {
-  return (uint32_t)(a.t1 - a.t0) * (uint32_t)a.tiles_i * (uint32_t)a.tiles_j;
-}
 
 template <typename T, bool SUN, bool VOX, bool FAST>
 void launch_siddon_f(const KArgs<T> &a, uint32_t blocks, hipStream_t s, bool bwd) {
@@ -392,24 +388,27 @@ int run_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, hipStream
     if (ray_i >= 0) return set_error(TG_EINVAL, "deterministic mode applies to full/image calls");
     return run_deterministic<T>(c, a, s);
   }
-  uint32_t blocks = grid_blocks(a);
+  const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
+  const bool vox = (c->voxmask != nullptr) || (c->vox_weight != nullptr && !bwd);
+  auto launch = [&](const KArgs<T> &av, uint32_t blocks) {
+    if (sun) {
+      if (vox) launch_siddon<T, true, true>(av, blocks, s, bwd);
+      else launch_siddon<T, true, false>(av, blocks, s, bwd);
+    } else {
+      if (vox) launch_siddon<T, false, true>(av, blocks, s, bwd);
+      else launch_siddon<T, false, false>(av, blocks, s, bwd);
+    }
+  };
   if (ray_i >= 0) {
     if (ray_i >= c->n1 || ray_j >= c->n2 || ray_i < 0 || ray_j < 0)
       return set_error(TG_EINVAL, "pixel (%d, %d) outside the %d x %d image", ray_i, ray_j, c->n1,
                        c->n2);
     a.ray_i = ray_i;
     a.ray_j = ray_j;
-    blocks = 1;
-  }
-  if (blocks == 0) return TG_OK;
-  const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
-  const bool vox = (c->voxmask != nullptr) || (c->vox_weight != nullptr && !bwd);
-  if (sun) {
-    if (vox) launch_siddon<T, true, true>(a, blocks, s, bwd);
-    else launch_siddon<T, true, false>(a, blocks, s, bwd);
-  } else {
-    if (vox) launch_siddon<T, false, true>(a, blocks, s, bwd);
-    else launch_siddon<T, false, false>(a, blocks, s, bwd);
+    if (a.t1 <= a.t0) return TG_OK;
+    launch(a, 1u);
+  } else if ((rc = for_view_launches(a, launch))) {
+    return rc;
   }
   if ((rc = hip_check(hipGetLastError(), "siddon kernel launch"))) return rc;
   return finish_acc<T>(a, s);
@@ -430,12 +429,15 @@ int run_count(const tg_call *c, uint32_t *per_ray, unsigned long long *total, hi
   if (rc) return rc;
   a.per_ray = per_ray;
   a.total = total;
-  const uint32_t blocks = grid_blocks(a);
-  if (blocks == 0) return TG_OK;
-  if (c->flags & TG_OBSTACLE_SUN)
-    hipLaunchKernelGGL((k_count<T, true>), dim3(blocks), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_count<T, false>), dim3(blocks), dim3(256), 0, s, a);
+  const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
+  rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
+    KArgs<T> ac = av;
+    // k_count writes per_ray relative to its launch's first view
+    if (per_ray) ac.per_ray = per_ray + (size_t)(av.t0 - a.t0) * (size_t)a.n1 * (size_t)a.n2;
+    if (sun) hipLaunchKernelGGL((k_count<T, true>), dim3(blocks), dim3(256), 0, s, ac);
+    else hipLaunchKernelGGL((k_count<T, false>), dim3(blocks), dim3(256), 0, s, ac);
+  });
+  if (rc) return rc;
   return hip_check(hipGetLastError(), "count kernel launch");
 }
 
@@ -445,9 +447,10 @@ int run_unit(const tg_call *c, int t0, int t1, void *u, hipStream_t s) {
   int rc = make_args<T>(c, t0, t1, a);
   if (rc) return rc;
   if (!u) return set_error(TG_EINVAL, "null u");
-  const uint32_t blocks = grid_blocks(a);
-  if (blocks == 0) return TG_OK;
-  hipLaunchKernelGGL((k_unit_vector<T>), dim3(blocks), dim3(256), 0, s, a, (T *)u);
+  rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
+    hipLaunchKernelGGL((k_unit_vector<T>), dim3(blocks), dim3(256), 0, s, av, (T *)u);
+  });
+  if (rc) return rc;
   return hip_check(hipGetLastError(), "unit vector kernel launch");
 }
 
@@ -457,10 +460,11 @@ int run_inter(const tg_call *c, int t0, int t1, const void *u, void *a1, void *a
   int rc = make_args<T>(c, t0, t1, a);
   if (rc) return rc;
   if (!u || !a1 || !an) return set_error(TG_EINVAL, "null u/a1/an");
-  const uint32_t blocks = grid_blocks(a);
-  if (blocks == 0) return TG_OK;
-  hipLaunchKernelGGL((k_intersections<T>), dim3(blocks), dim3(256), 0, s, a, (const T *)u, (T *)a1,
-                     (T *)an);
+  rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
+    hipLaunchKernelGGL((k_intersections<T>), dim3(blocks), dim3(256), 0, s, av, (const T *)u,
+                       (T *)a1, (T *)an);
+  });
+  if (rc) return rc;
   return hip_check(hipGetLastError(), "intersection kernel launch");
 }
 

@@ -328,8 +328,16 @@ int run_deterministic(const tg_call *c, const KArgs<T> &a0, hipStream_t s) {
   const int64_t nvox = L.ngroups * kGroup;
   const int64_t row_end = (int64_t)a0.t1 * a0.n1;
   // ray keys are 32-bit: (row - row0) * n2 + j
-  const int64_t max_rows = (int64_t)(0xffffffffull / (uint64_t)(a0.n2 > 0 ? a0.n2 : 1));
+  int64_t max_rows = (int64_t)(0xffffffffull / (uint64_t)(a0.n2 > 0 ? a0.n2 : 1));
   if (max_rows < 1) return set_error(TG_EINVAL, "image too wide for 32-bit ray keys");
+  // the count pass launches every view a chunk touches in one grid; a chunk of
+  // rows touches at most rows / n1 + 2 views
+  const uint64_t per_view_groups = (uint64_t)a0.tiles_i * (uint64_t)a0.tiles_j;
+  if (per_view_groups > kMaxGroups256 / 2)
+    return set_error(TG_EINVAL, "a %d x %d image is too large for deterministic mode", a0.n1, a0.n2);
+  const int64_t view_cap = (int64_t)(kMaxGroups256 / per_view_groups) - 2;
+  if (view_cap < 1) max_rows = 1;
+  else if (view_cap * (int64_t)a0.n1 < max_rows) max_rows = view_cap * (int64_t)a0.n1;
   // first chunk: what fits even if every ray crossed the whole cube
   const size_t per_row_bound = det_row_records(c) / kGroup;
   int64_t chunk = (int64_t)(cap / (per_row_bound > 0 ? per_row_bound : 1));

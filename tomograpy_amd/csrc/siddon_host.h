@@ -28,6 +28,32 @@ int finish_acc_table(const void *dev_table, size_t stride, const KArgs<T> *const
 template <typename T>
 int run_deterministic(const tg_call *c, const KArgs<T> &a, hipStream_t s);
 
+// An HSA dispatch holds its grid size in work-items as 32 bits, so a launch
+// of 256-lane workgroups has at most this many workgroups.
+constexpr uint64_t kMaxGroups256 = 0xffffffffull / 256ull;
+
+// Calls launch(view_args, groups) for consecutive view ranges of [a.t0, a.t1)
+// whose ray-tile grids (256-lane workgroups, tiles_i x tiles_j per view) each
+// fit one dispatch; view_args is `a` with t0/t1 narrowed to the range.
+template <typename T, typename F>
+int for_view_launches(const KArgs<T> &a, F &&launch) {
+  const uint64_t per_view = (uint64_t)a.tiles_i * (uint64_t)a.tiles_j;
+  if (per_view == 0 || a.t1 <= a.t0) return TG_OK;
+  if (per_view > kMaxGroups256)
+    return set_error(TG_EINVAL, "a %d x %d image needs %llu workgroups, more than one launch holds",
+                     a.n1, a.n2, (unsigned long long)per_view);
+  const uint64_t fit = kMaxGroups256 / per_view;
+  const int step = fit < (uint64_t)(a.t1 - a.t0) ? (int)fit : a.t1 - a.t0;
+  for (int v = a.t0; v < a.t1; v += step) {
+    KArgs<T> av = a;
+    av.t0 = v;
+    av.t1 = a.t1 - v < step ? a.t1 : v + step;
+    launch(av, (uint32_t)((uint64_t)(av.t1 - av.t0) * per_view));
+    if (a.t1 - v <= step) break;
+  }
+  return TG_OK;
+}
+
 // Storage extent (max element offset + 1) a call can touch in the cube.
 inline int64_t cube_extent(const tg_call *c) {
   const bool time = (c->kind == TG_PJT || c->kind == TG_BPJT);

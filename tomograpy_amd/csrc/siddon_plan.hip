@@ -539,13 +539,15 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
   char *meta = (char *)p->meta;
   uint32_t *counts = (uint32_t *)(meta + m.counts);
   hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, s, counts, g.nbricks);
-  const uint32_t blocks = (uint32_t)c->nv * a.tiles_i * a.tiles_j;
-  if (blocks) {
-    if (c->flags & TG_OBSTACLE_SUN)
-      hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, a, g, counts);
+  static_assert(kPlanBlock == 256, "view-range launches assume 256-lane workgroups");
+  const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
+  rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
+    if (sun)
+      hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g, counts);
     else
-      hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, a, g, counts);
-  }
+      hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g, counts);
+  });
+  if (rc) return rc;
   if ((rc = hip_check(hipGetLastError(), "plan count launch"))) return rc;
   std::vector<uint32_t> hc(g.nbricks);
   if ((rc = hip_check(hipMemcpyAsync(hc.data(), counts, 4 * (size_t)g.nbricks,
@@ -623,16 +625,17 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   const unsigned long long *offsets = (const unsigned long long *)(meta + m.offsets);
   const uint32_t *mult = (const uint32_t *)(meta + m.mult);
   hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, s, cursor, g.nbricks);
-  const uint32_t blocks = (uint32_t)c->nv * a.tiles_i * a.tiles_j;
   RecPtrs<T> rp = rec_ptrs<T>(p);
-  if (blocks) {
-    if (c->flags & TG_OBSTACLE_SUN)
-      hipLaunchKernelGGL((k_plan_fill<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, a, g,
+  const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
+  rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
+    if (sun)
+      hipLaunchKernelGGL((k_plan_fill<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
                          offsets, cursor, mult, rp);
     else
-      hipLaunchKernelGGL((k_plan_fill<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, a, g,
+      hipLaunchKernelGGL((k_plan_fill<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
                          offsets, cursor, mult, rp);
-  }
+  });
+  if (rc) return rc;
   if ((rc = hip_check(hipGetLastError(), "plan fill launch"))) return rc;
   if (g.sort && p->chunks > 0) {
     hipLaunchKernelGGL((k_plan_sort<T>), dim3((uint32_t)p->chunks), dim3(kBpjBlock), 0, s, g,

@@ -102,7 +102,8 @@ typedef struct tg_call {
   const uint32_t *ray_steps; /* device or NULL, pj/pjt: the Siddon step count of every ray,
                                 indexed (t * n1 + i) * n2 + j, as tg_count_steps writes it for
                                 this geometry (mask and obstacle included); lets the projector
-                                run each ray's loop for a known count */
+                                run each ray's loop for a known count (ignored for stacks of
+                                2^32 rays or more) */
 } tg_call;
 
 /* Size in bytes of the packed view descriptors for nv views of n1 x n2 pixels. */

@@ -266,7 +266,11 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a, int tile_j = 0) {
   a.voxw = (const T *)c->vox_weight;
   a.voxmask = c->voxmask;
   a.status = c->status;
-  a.ray_steps = (c->kind == TG_PJ || c->kind == TG_PJT) ? c->ray_steps : nullptr;
+  // step counts are indexed in 32 bits; larger stacks walk each ray to its end
+  a.ray_steps = ((c->kind == TG_PJ || c->kind == TG_PJT) &&
+                 (uint64_t)c->nv * (uint64_t)c->n1 * (uint64_t)c->n2 <= 0xffffffffull)
+                    ? c->ray_steps
+                    : nullptr;
   a.cube_bytes = buffer_walk_bytes(a.n, a.cs, (int)sizeof(T));
   {
     const int64_t dn[3] = {c->n1, c->n2, c->nv};

@@ -183,6 +183,8 @@ class Siddon:
             return True
         if max(self.cube_shape) > 1024:
             return False
+        if self.n1 * self.n2 * self.nv >= 1 << 32:
+            return False  # the plan's ray ids are 32-bit
         if self._auto_plan is None:
             steps = self.count_steps()
             voxels = int(np.prod(self.cube_shape)) * getattr(self, "nt", 1)

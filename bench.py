@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-shard", action="store_true",
                     help="skip the leg that times one GPU's share of a view-sharded run")
+    ap.add_argument("--no-det", action="store_true",
+                    help="skip the deterministic (fixed summation order) backward leg")
     return ap.parse_args()
 
 
@@ -352,6 +354,23 @@ def shard_leg(args, nshards=8, reps=10):
             "siddon_steps_per_s_slowest_shard": 2 * steps[k] / (ms[k] * 1e-3)}
 
 
+def det_leg(args, reps=2):
+    """The deterministic backward (bit-identical to the reference's single-thread
+    order) at the workload's shape, against the operator's default backward."""
+    import torch
+    from tomograpy_amd import workloads as W
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    dtype = np.dtype(args.dtype)
+    ch, hs, rng = W.config2_geometry(dtype, n=args.n, n_views=args.views, n_pix=args.pix)
+    y = torch.from_numpy(rng.normal(size=(args.pix, args.pix, len(hs))).astype(dtype)).cuda()
+    Pd = siddon_lo(hs, ch, dtype=dtype, deterministic=True)
+    ms = _event_ms(lambda: Pd.rmatvec(y), reps)
+    out = {"ms_backward": ms, "workspace_bytes": int(Pd._ws.numel()) if Pd._ws is not None else 0}
+    del Pd
+    torch.cuda.empty_cache()
+    return out
+
+
 def measured_traffic(kernel, args, plan_info):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary, when this run is the workload that summary measured
@@ -537,7 +556,7 @@ def main():
 
     y_host = y.cpu().numpy() if not args.no_cpu else None
     if args.serve_requests or not args.no_grad or not args.no_batch or args.cg_iters \
-            or not args.no_shard:
+            or not args.no_shard or not args.no_det:
         # release the pair's operator (its brick plan is 9.4 GB at 256^3) first
         del P, x, x_true, y, noise
         torch.cuda.empty_cache()
@@ -555,6 +574,8 @@ def main():
     if not args.no_shard:
         extra["shard_of_8"] = shard_leg(args)
         torch.cuda.empty_cache()
+    if not args.no_det:
+        extra["deterministic"] = det_leg(args)
     if args.serve_requests:
         extra["serving"] = serving_leg(args)
         extra["serving_small"] = serving_leg(args, small=True)

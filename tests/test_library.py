@@ -36,6 +36,18 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert b"gfx950" in lib.tg_version()
 
 
+def test_missing_library_raises_instead_of_falling_back(monkeypatch):
+    """Without the HIP library the package refuses to compute: no CPU path."""
+    from tomograpy_amd import siddon as S
+    monkeypatch.setattr(N, "_lib", None)
+    monkeypatch.setattr(N, "LIB_PATH", os.path.join(ROOT, "no-such-dir", "libtomograpy_amd.so"))
+    with pytest.raises(ImportError, match="missing"):
+        N.lib()
+    # a product helper reaches the library before any device work
+    with pytest.raises(ImportError, match="missing"):
+        S.views_tensor(np.zeros((1, 18)), N.TG_F32, 4, 4, "cpu")
+
+
 def test_struct_layout_matches_header():
     # sizes of the POD structs as the C compiler lays them out
     assert ctypes.sizeof(N.TgMap) == 12 * 8 + 4 * 4 + 4 * 8

@@ -104,6 +104,25 @@ __device__ __forceinline__ uint32_t brick_local(const BrickGeom &g, const int iv
          ((uint32_t)(iv[1] & (g.b[1] - 1)) << g.lg[2]) | (uint32_t)(iv[2] & (g.b[2] - 1));
 }
 
+// LDS layout of a brick's accumulators: axis 2 fastest, each axis-2 row
+// padded by one cell and each axis-1 plane by one more.  With power-of-two
+// pitches, voxels that differ only along axis 0 or 1 share an LDS bank, and
+// the neighbouring rays of a wavefront hit exactly such voxels together.
+__host__ __device__ __forceinline__ int brick_pitch1(const BrickGeom &g) { return g.b[2] + 1; }
+__host__ __device__ __forceinline__ int brick_pitch0(const BrickGeom &g) {
+  return g.b[1] * (g.b[2] + 1) + 1;
+}
+__host__ __device__ __forceinline__ int brick_lds_cells(const BrickGeom &g) {
+  return g.b[0] * brick_pitch0(g);
+}
+// LDS cell of the brick-local index l (the unpadded index brick_local writes)
+__device__ __forceinline__ int brick_cell(const BrickGeom &g, uint32_t l) {
+  const int l2 = (int)(l & (uint32_t)(g.b[2] - 1));
+  const int l1 = (int)((l >> g.lg[2]) & (uint32_t)(g.b[1] - 1));
+  const int l0 = (int)(l >> (g.lg[1] + g.lg[2]));
+  return l0 * brick_pitch0(g) + l1 * brick_pitch1(g) + l2;
+}
+
 __device__ __forceinline__ int brick_of(const BrickGeom &g, const int iv[3], int bin) {
   return ((((iv[0] >> g.lg[0]) * g.nb[1]) + (iv[1] >> g.lg[1])) * g.nb[2] + (iv[2] >> g.lg[2])) +
          bin * g.nb[0] * g.nb[1] * g.nb[2];
@@ -338,7 +357,8 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
   const int bs = b % nsp;
   const int bz = bs % g.nb[2], by = (bs / g.nb[2]) % g.nb[1], bx = bs / (g.nb[2] * g.nb[1]);
   const int lo0 = bx << g.lg[0], lo1 = by << g.lg[1], lo2 = bz << g.lg[2];
-  for (int k = threadIdx.x; k < vol; k += blockDim.x) acc[k] = (Acc)0;
+  const int cells = brick_lds_cells(g);
+  for (int k = threadIdx.x; k < cells; k += blockDim.x) acc[k] = (Acc)0;
   __syncthreads();
   // The record loop is uniform per wavefront (every lane takes part in the
   // ballot below); lanes past the chunk's end walk no steps.
@@ -366,10 +386,10 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
     // wavefront together and leaves every sum unchanged
     // byte offsets in the double accumulators of one step along each axis
     const int u0 = ((e.up >> 0) & 3) - 1, u1 = ((e.up >> 2) & 3) - 1, u2 = ((e.up >> 4) & 3) - 1;
-    const int dm0 = u0 * (int)(sizeof(Acc) << (g.lg[1] + g.lg[2]));
-    const int dm1 = u1 * (int)(sizeof(Acc) << g.lg[2]);
+    const int dm0 = u0 * brick_pitch0(g) * (int)sizeof(Acc);
+    const int dm1 = u1 * brick_pitch1(g) * (int)sizeof(Acc);
     const int dm2 = u2 * (int)sizeof(Acc);
-    unsigned char *cell = (unsigned char *)(acc + (lq & 0xffffu));
+    unsigned char *cell = (unsigned char *)(acc + brick_cell(g, lq & 0xffffu));
     const uint32_t n = valid ? lq >> 16 : 0u;
     auto step = [&]() {
       const T d = tmin(tmin(D0, D1), D2);
@@ -390,9 +410,9 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
   __syncthreads();
   const int64_t boff = (int64_t)bin * a.cs[3];
   for (int k = threadIdx.x; k < vol; k += blockDim.x) {
-    const Acc sum = acc[k];
-    if (sum == (Acc)0) continue;
     const int l2 = k & (g.b[2] - 1), l1 = (k >> g.lg[2]) & (g.b[1] - 1), l0 = k >> (g.lg[2] + g.lg[1]);
+    const Acc sum = acc[l0 * brick_pitch0(g) + l1 * brick_pitch1(g) + l2];
+    if (sum == (Acc)0) continue;
     const int64_t i0 = lo0 + l0, i1 = lo1 + l1, i2 = lo2 + l2;
     if (a.voxmask && a.voxmask[i0 * a.vs[0] + i1 * a.vs[1] + i2 * a.vs[2]]) continue;
     const int64_t off = boff + i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2];
@@ -660,7 +680,7 @@ template <typename T>
 int launch_plan_bpj(const KArgs<T> &a, const BrickGeom &g, const tg_bpj_plan *p, hipStream_t s) {
   const MetaLayout m = meta_layout(g.nbricks);
   const char *meta = (const char *)p->meta;
-  const size_t lds = sizeof(Acc) * (size_t)g.b[0] * g.b[1] * g.b[2];
+  const size_t lds = sizeof(Acc) * (size_t)brick_lds_cells(g);
   static bool attr_set = false;
   int rc = allow_full_lds(k_plan_bpj<T>, attr_set);
   if (rc) return rc;
@@ -684,7 +704,7 @@ int plan_bpj(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   if (!p->meta || p->meta_bytes < meta_layout(g.nbricks).total || !p->rec ||
       p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
     return set_error(TG_EINVAL, "plan buffers missing or too small");
-  const size_t lds = sizeof(Acc) * (size_t)g.b[0] * g.b[1] * g.b[2];
+  const size_t lds = sizeof(Acc) * (size_t)brick_lds_cells(g);
   if (lds > 160 * 1024) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
   if (p->chunks == 0) return TG_OK;
   // the Sun's effect is in the records' step counts: one kernel for both obstacles
@@ -711,7 +731,7 @@ int plan_job(const tg_call *c, const tg_bpj_plan *p, PlanJob<T> &j, size_t &lds)
   if (!p->meta || p->meta_bytes < meta_layout(j.g.nbricks).total || !p->rec ||
       p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
     return set_error(TG_EINVAL, "plan buffers missing or too small");
-  lds = sizeof(Acc) * (size_t)j.g.b[0] * j.g.b[1] * j.g.b[2];
+  lds = sizeof(Acc) * (size_t)brick_lds_cells(j.g);
   if (lds > 160 * 1024) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
   const MetaLayout m = meta_layout(j.g.nbricks);
   const char *meta = (const char *)p->meta;

@@ -308,6 +308,26 @@ def test_plan_backprojection_c1(oracle, dtype, acc64, obstacle):
     assert rel_l2(out, ref) <= (1e-5 if dtype == np.float32 else 1e-12)
 
 
+def test_plan_backprojection_any_stack_layout():
+    """A stack in the reference's (n1, n2, views) C order, a view-major one and a
+    strided view give the same cube (float64 sums do not depend on the order)."""
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    print("plan bpj stack layouts", flush=True)
+    ch, cube, data = c1_problem(np.float32)
+    rng = np.random.default_rng(22)
+    y = rng.uniform(-1, 1, data.shape).astype(np.float32)
+    P = siddon_lo(data.header, ch, dtype=np.float32, bpj_mode="plan")
+    yc = torch.from_numpy(y).cuda()  # views last, C order
+    yv = torch.from_numpy(np.ascontiguousarray(y.transpose(2, 0, 1))).cuda().permute(1, 2, 0)
+    big = torch.zeros((y.shape[0], 2 * y.shape[1], y.shape[2]), device="cuda")
+    big[:, ::2] = yc
+    ys = big[:, ::2]  # strided
+    outs = [P.rmatvec(t).clone() for t in (yv, yc, ys, yv)]
+    T.check_status()
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+
+
 # ---- float64 accumulation of float32 backprojections (TG_ACC_F64) ----------------------
 def assert_ulp_close(a, b, max_frac=1e-3):
     """Equal except, in at most max_frac of the entries, by one unit in the last place."""

@@ -377,7 +377,24 @@ class Siddon:
         self._backproject_raw(y, out)
         return out
 
+    def _view_major(self, y):
+        """y itself if it is stored view-major (the operator's own layout), else
+        a copy into a cached buffer of that layout.  The brick plan gathers one
+        pixel per record in plan order, and the reference's (n1, n2, views)
+        C order spreads a view's pixels views-apart: 10.0 instead of 6.4 ms
+        for a 256^3 x 128 x 512^2 backprojection, against 0.1 ms for the copy."""
+        if tuple(y.stride()) == (self.n2, 1, self.n1 * self.n2):
+            return y
+        buf = getattr(self, "_ybuf", None)
+        if buf is None:
+            buf = stack_buffer(self.n1, self.n2, self.nv, self.tdtype, self.dev)
+            self._ybuf = buf
+        buf.copy_(y)
+        return buf
+
     def _backproject_raw(self, y, out):
+        if self.use_plan():
+            y = self._view_major(y)
         c = self._call(N.TG_BPJ, out, y)
         if self.use_plan():
             p = self.build_plan()

@@ -175,7 +175,9 @@ int tg_vox_apply(const tg_call *c, const void *src, const int64_t src_stride[4],
  * in the atomic mode). */
 typedef struct tg_bpj_plan {
   int32_t brick[3];      /* brick edge per axis, powers of two; 0,0,0 = default: 32 x 16 x 32
-                            (128 KiB), or 16 x 16 x 32 for cubes of edge <= 96 */
+                            (132 KiB of LDS), or 16 x 16 x 32 for cubes of edge <= 96; a
+                            brick's float64 accumulators, b0 (b1 (b2 + 1) + 1) of them,
+                            must fit 160 KiB */
   int32_t chunk_records; /* records per workgroup chunk; 0 = chosen by tg_bpj_plan_count
                             (about 1024 chunks of 2K..64K records) and written back */
   int32_t nb_axis[3];    /* bricks per axis (set by tg_bpj_plan_init) */

@@ -34,6 +34,8 @@ int make_args(const tg_call *c, int t0, int t1, KArgs<T> &a, int tile_j = 0);
 namespace {
 
 constexpr int kPlanBlock = 256;
+// LDS one backprojection workgroup may allocate (gfx950: 160 KiB per CU)
+constexpr int kPlanLdsBytes = 160 * 1024;
 constexpr int kHistMax = 8192;  // bricks x bins histogrammed in LDS; above: global atomics
 constexpr int kBpjBlock = 1024;
 static_assert(kBpjBlock == 1024, "the plan sort keys hold a 10-bit pass position");
@@ -99,10 +101,6 @@ struct RecPtrs {
   RayEntry<T> *ray;  // indexed by (t * n1 + i) * n2 + j
 };
 
-__device__ __forceinline__ uint32_t brick_local(const BrickGeom &g, const int iv[3]) {
-  return ((uint32_t)(iv[0] & (g.b[0] - 1)) << (g.lg[1] + g.lg[2])) |
-         ((uint32_t)(iv[1] & (g.b[1] - 1)) << g.lg[2]) | (uint32_t)(iv[2] & (g.b[2] - 1));
-}
 
 // LDS layout of a brick's accumulators: axis 2 fastest, each axis-2 row
 // padded by one cell and each axis-1 plane by one more.  With power-of-two
@@ -112,15 +110,15 @@ __host__ __device__ __forceinline__ int brick_pitch1(const BrickGeom &g) { retur
 __host__ __device__ __forceinline__ int brick_pitch0(const BrickGeom &g) {
   return g.b[1] * (g.b[2] + 1) + 1;
 }
-__host__ __device__ __forceinline__ int brick_lds_cells(const BrickGeom &g) {
-  return g.b[0] * brick_pitch0(g);
+__host__ __device__ __forceinline__ long long brick_lds_cells(const BrickGeom &g) {
+  return (long long)g.b[0] * brick_pitch0(g);
 }
-// LDS cell of the brick-local index l (the unpadded index brick_local writes)
-__device__ __forceinline__ int brick_cell(const BrickGeom &g, uint32_t l) {
-  const int l2 = (int)(l & (uint32_t)(g.b[2] - 1));
-  const int l1 = (int)((l >> g.lg[2]) & (uint32_t)(g.b[1] - 1));
-  const int l0 = (int)(l >> (g.lg[1] + g.lg[2]));
-  return l0 * brick_pitch0(g) + l1 * brick_pitch1(g) + l2;
+
+// The LDS cell of voxel iv inside its brick: what a record stores in the low
+// 16 bits of lq.
+__device__ __forceinline__ uint32_t brick_local(const BrickGeom &g, const int iv[3]) {
+  return (uint32_t)((iv[0] & (g.b[0] - 1)) * brick_pitch0(g) +
+                    (iv[1] & (g.b[1] - 1)) * brick_pitch1(g) + (iv[2] & (g.b[2] - 1)));
 }
 
 __device__ __forceinline__ int brick_of(const BrickGeom &g, const int iv[3], int bin) {
@@ -357,7 +355,7 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
   const int bs = b % nsp;
   const int bz = bs % g.nb[2], by = (bs / g.nb[2]) % g.nb[1], bx = bs / (g.nb[2] * g.nb[1]);
   const int lo0 = bx << g.lg[0], lo1 = by << g.lg[1], lo2 = bz << g.lg[2];
-  const int cells = brick_lds_cells(g);
+  const int cells = (int)brick_lds_cells(g);
   for (int k = threadIdx.x; k < cells; k += blockDim.x) acc[k] = (Acc)0;
   __syncthreads();
   // The record loop is uniform per wavefront (every lane takes part in the
@@ -389,7 +387,7 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
     const int dm0 = u0 * brick_pitch0(g) * (int)sizeof(Acc);
     const int dm1 = u1 * brick_pitch1(g) * (int)sizeof(Acc);
     const int dm2 = u2 * (int)sizeof(Acc);
-    unsigned char *cell = (unsigned char *)(acc + brick_cell(g, lq & 0xffffu));
+    unsigned char *cell = (unsigned char *)(acc + (lq & 0xffffu));
     const uint32_t n = valid ? lq >> 16 : 0u;
     auto step = [&]() {
       const T d = tmin(tmin(D0, D1), D2);
@@ -481,9 +479,12 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
     g.nb[k] = (c->map.n[k] + b - 1) / b;
     if (c->map.n[k] > 1024) return set_error(TG_EINVAL, "plan backprojection supports n <= 1024");
   }
-  // a record packs the voxel index inside the brick and the steps in it in 16 bits each
-  if ((long long)g.b[0] * g.b[1] * g.b[2] > 65536 || g.b[0] + g.b[1] + g.b[2] > 65535)
-    return set_error(TG_EINVAL, "brick of %d x %d x %d voxels is too large", g.b[0], g.b[1], g.b[2]);
+  // a record packs the voxel's LDS cell and the steps in the brick in 16 bits
+  // each, and the brick's float64 accumulators must fit one workgroup's LDS
+  if (brick_lds_cells(g) * (long long)sizeof(Acc) > kPlanLdsBytes ||
+      g.b[0] + g.b[1] + g.b[2] > 65535)
+    return set_error(TG_EINVAL, "brick of %d x %d x %d voxels exceeds the %d bytes of LDS", g.b[0],
+                     g.b[1], g.b[2], kPlanLdsBytes);
   const long long nb = (long long)g.nb[0] * g.nb[1] * g.nb[2] * g.nt;
   if (nb > (1 << 30)) return set_error(TG_EINVAL, "too many bricks");
   g.nbricks = (int)nb;
@@ -670,7 +671,7 @@ template <class K>
 int allow_full_lds(K kernel, bool &done) {
   if (done) return TG_OK;
   hipError_t e = hipFuncSetAttribute((const void *)kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, kPlanLdsBytes);
   if (e != hipSuccess) return hip_check(e, "plan bpj LDS attribute");
   done = true;
   return TG_OK;
@@ -705,7 +706,7 @@ int plan_bpj(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
       p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
     return set_error(TG_EINVAL, "plan buffers missing or too small");
   const size_t lds = sizeof(Acc) * (size_t)brick_lds_cells(g);
-  if (lds > 160 * 1024) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
+  if (lds > (size_t)kPlanLdsBytes) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
   if (p->chunks == 0) return TG_OK;
   // the Sun's effect is in the records' step counts: one kernel for both obstacles
   if ((rc = launch_plan_bpj<T>(a, g, p, s))) return rc;
@@ -732,7 +733,7 @@ int plan_job(const tg_call *c, const tg_bpj_plan *p, PlanJob<T> &j, size_t &lds)
       p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
     return set_error(TG_EINVAL, "plan buffers missing or too small");
   lds = sizeof(Acc) * (size_t)brick_lds_cells(j.g);
-  if (lds > 160 * 1024) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
+  if (lds > (size_t)kPlanLdsBytes) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
   const MetaLayout m = meta_layout(j.g.nbricks);
   const char *meta = (const char *)p->meta;
   j.offsets = (const unsigned long long *)(meta + m.offsets);

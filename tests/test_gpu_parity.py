@@ -326,6 +326,15 @@ def test_plan_backprojection_any_stack_layout():
     T.check_status()
     for o in outs[1:]:
         assert torch.equal(o, outs[0])
+    # the C ABI reads any stack layout itself (ray ids decoded to pixel offsets)
+    import ctypes
+    from tomograpy_amd import _native as N, siddon as S
+    for t in (yc, ys):
+        out = torch.zeros_like(outs[0])
+        c = P._call(N.TG_BPJ, out, t)
+        N.check(N.lib().tg_bpj_plan_backproject(ctypes.byref(c), ctypes.byref(P.build_plan()),
+                                                S.stream_handle()))
+        assert torch.equal(out, outs[0])
 
 
 # ---- float64 accumulation of float32 backprojections (TG_ACC_F64) ----------------------

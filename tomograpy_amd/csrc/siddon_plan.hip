@@ -54,31 +54,24 @@ struct BrickGeom {
   int32_t nbricks;    // nb0 * nb1 * nb2 * nt
   int32_t nt;
   int32_t chunk;      // records per chunk
-  int32_t packed;     // ray ids packed as t << 20 | i << 10 | j (else (t n1 + i) n2 + j)
   int32_t shuffle;    // records of a brick stored in a permuted order
   int32_t sort;       // records of each workgroup pass sorted by their step counts
   int32_t spread;     // low bits of the pass position reversed to order ties
 };
 
-__device__ __forceinline__ uint32_t ray_id(const BrickGeom &g, uint32_t n1, uint32_t n2, int t, int i,
-                                           int j) {
-  return g.packed ? (((uint32_t)t << 20) | ((uint32_t)i << 10) | (uint32_t)j)
-                  : (((uint32_t)t * n1 + (uint32_t)i) * n2 + (uint32_t)j);
+// A record's ray id is the ray's linear index (t n1 + i) n2 + j: the index of
+// its ray-table entry, and of its pixel in a view-major image stack.
+__device__ __forceinline__ uint32_t ray_id(uint32_t n1, uint32_t n2, int t, int i, int j) {
+  return ((uint32_t)t * n1 + (uint32_t)i) * n2 + (uint32_t)j;
 }
 
-__device__ __forceinline__ void ray_pixel(const BrickGeom &g, uint32_t n1, uint32_t n2, uint32_t rid,
-                                          int &t, int &i, int &j) {
-  if (g.packed) {
-    t = (int)(rid >> 20);
-    i = (int)((rid >> 10) & 1023u);
-    j = (int)(rid & 1023u);
-  } else {
-    const uint32_t per_view = n1 * n2;
-    t = (int)(rid / per_view);
-    const uint32_t rem = rid % per_view;
-    i = (int)(rem / n2);
-    j = (int)(rem % n2);
-  }
+__device__ __forceinline__ void ray_pixel(uint32_t n1, uint32_t n2, uint32_t rid, int &t, int &i,
+                                          int &j) {
+  const uint32_t per_view = n1 * n2;
+  t = (int)(rid / per_view);
+  const uint32_t rem = rid % per_view;
+  i = (int)(rem / n2);
+  j = (int)(rem % n2);
 }
 
 // Per-ray constants of C:750-774 (|p| = |CDELT / u| and the step direction
@@ -153,7 +146,7 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
       return;
     }
   }
-  const uint32_t rid = ray_id(g, (uint32_t)a.n1, (uint32_t)a.n2, t, i, j);
+  const uint32_t rid = ray_id((uint32_t)a.n1, (uint32_t)a.n2, t, i, j);
   int last = -1;
   uint32_t seg = 0;
   int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
@@ -224,14 +217,12 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
   uint32_t lcur = 0;
   trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &r, uint32_t rid, bool first) {
     if (first) {
-      int t, i, j;
-      ray_pixel(g, (uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
       RayEntry<T> e;
       e.pabs[0] = r.pabs[0];
       e.pabs[1] = r.pabs[1];
       e.pabs[2] = r.pabs[2];
       e.up = (r.up[0] + 1) | ((r.up[1] + 1) << 2) | ((r.up[2] + 1) << 4);
-      rec.ray[((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j] = e;
+      rec.ray[rid] = e;
     }
     const uint32_t s = lds ? atomicAdd(&slot[b], 1u) : atomicAdd(&cursor[b], 1u);
     unsigned long long q = s;
@@ -362,6 +353,9 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
   // ballot below); lanes past the chunk's end walk no steps.
   const uint32_t lane = threadIdx.x & 63u;
   const unsigned long long wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x - lane);
+  // a view-major stack holds the pixel of ray id r at r: no decoding
+  const bool view_major = a.ds[1] == 1 && a.ds[0] == (int64_t)a.n2 &&
+                          a.ds[2] == (int64_t)a.n1 * (int64_t)a.n2;
   for (unsigned long long pw = r0 + wave0; pw < r1; pw += blockDim.x) {
     const bool valid = pw + lane < r1;
     const unsigned long long p = valid ? pw + lane : r1 - 1;
@@ -370,15 +364,16 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
     const uint32_t rid = rec.rid[p];
     const uint32_t lq = rec.lq[p];
     T D0 = rec.d0[p], D1 = rec.d1[p], D2 = rec.d2[p];
-    int t, i, j;
-    ray_pixel(g, (uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
-    const int64_t did = a.data32 ? (int64_t)((uint32_t)i * (uint32_t)a.ds[0] +
-                                             (uint32_t)j * (uint32_t)a.ds[1] +
-                                             (uint32_t)t * (uint32_t)a.ds[2])
-                                 : i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+    int64_t did = rid;
+    if (!view_major) {
+      int t, i, j;
+      ray_pixel((uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
+      did = a.data32 ? (int64_t)((uint32_t)i * (uint32_t)a.ds[0] + (uint32_t)j * (uint32_t)a.ds[1] +
+                                 (uint32_t)t * (uint32_t)a.ds[2])
+                     : i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+    }
     T w = a.data[did];
-    const RayEntry<T> e =
-        rec.ray[((uint32_t)t * (uint32_t)a.n1 + (uint32_t)i) * (uint32_t)a.n2 + (uint32_t)j];
+    const RayEntry<T> e = rec.ray[rid];
     if (a.pixw) w = a.pixw[did] * w;
     // a zero pixel value walks its steps adding zeros, which keeps the
     // wavefront together and leaves every sum unchanged
@@ -489,7 +484,6 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
   if (nb > (1 << 30)) return set_error(TG_EINVAL, "too many bricks");
   g.nbricks = (int)nb;
   g.chunk = p->chunk_records > 0 ? p->chunk_records : 32768;
-  g.packed = (c->n1 <= 1024 && c->n2 <= 1024 && c->nv <= 4096) ? 1 : 0;
   g.shuffle = p->order == 1 ? 1 : 0;
   g.sort = p->order == 2 ? 1 : 0;
   if (p->tie_spread < 0 || p->tie_spread > 10)

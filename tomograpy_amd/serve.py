@@ -27,6 +27,7 @@ import torch
 
 from . import _native as N
 from . import cg as CG
+from . import siddon as S
 from .lo_wrapper import Siddon, Siddon4d
 
 
@@ -59,6 +60,7 @@ class Request:
         self.t_done = None
         self._op = None
         self._reserved = 0
+        self._work = None
         self.x_host = None
 
     def footprint(self):
@@ -77,10 +79,12 @@ class Request:
 
     def work(self):
         """Relative cost of one CG iteration: rays x mean cube edge."""
-        n = sum(self.cube_header["NAXIS%d" % (k + 1)] for k in range(3)) / 3.0
-        n1 = self.data_header[0]["NAXIS1"]
-        n2 = self.data_header[0]["NAXIS2"]
-        return float(n1 * n2 * len(self.data_header) * n)
+        if self._work is None:
+            n = sum(self.cube_header["NAXIS%d" % (k + 1)] for k in range(3)) / 3.0
+            n1 = self.data_header[0]["NAXIS1"]
+            n2 = self.data_header[0]["NAXIS2"]
+            self._work = float(n1 * n2 * len(self.data_header) * n)
+        return self._work
 
     def result_shape(self):
         n = tuple(int(self.cube_header["NAXIS%d" % (k + 1)]) for k in range(3))
@@ -236,6 +240,10 @@ class Server:
 
     def run(self, requests):
         """Serves `requests` (arrival times in seconds from now); returns a summary."""
+        with S.fixed_stream():
+            return self._run(requests)
+
+    def _run(self, requests):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         active = []  # (req, solve, row)
@@ -353,6 +361,11 @@ def _collect(finishing, t0, block=False):
 
 def serve_plain(requests, check_every=5):
     """Per-request baseline: solve each request alone, in arrival order."""
+    with S.fixed_stream():
+        return _serve_plain(requests, check_every)
+
+
+def _serve_plain(requests, check_every):
     order = sorted(requests, key=lambda r: r.arrival)
     t0 = time.perf_counter()
     for req in order:

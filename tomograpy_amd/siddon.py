@@ -18,6 +18,7 @@ Arrays:
 (or (N0, N1, N2, T) for the 4-D variants) and ``cube.header`` a dict.
 """
 import collections
+import contextlib
 import ctypes
 
 import numpy as np
@@ -146,8 +147,30 @@ def make_call(kind, dtype_code, data_shape, data_strides, cube_shape, cube_strid
     return c
 
 
+# The stream handle fixed by fixed_stream(), else None.
+_FIXED_STREAM = None
+
+
 def stream_handle():
+    """The HIP stream the library's calls go to: torch's current stream."""
+    if _FIXED_STREAM is not None:
+        return _FIXED_STREAM
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@contextlib.contextmanager
+def fixed_stream():
+    """Inside the block, stream_handle() returns the stream current at entry
+    without asking torch again (about 9 us a call), for loops that make many
+    library calls on one stream.  The block must not switch streams, and other
+    threads must not call the library meanwhile."""
+    global _FIXED_STREAM
+    prev = _FIXED_STREAM
+    _FIXED_STREAM = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    try:
+        yield
+    finally:
+        _FIXED_STREAM = prev
 
 
 def det_workspace(call, dev, budget_bytes=None):

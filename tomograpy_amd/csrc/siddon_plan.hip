@@ -38,6 +38,8 @@ constexpr int kPlanBlock = 256;
 constexpr int kPlanLdsBytes = 160 * 1024;
 constexpr int kHistMax = 8192;  // bricks x bins histogrammed in LDS; above: global atomics
 constexpr int kBpjBlock = 1024;
+// chunks (kBpjBlock-lane workgroups) one dispatch holds: 2^32 - 1 work-items
+constexpr unsigned long long kMaxChunksPerLaunch = 0xffffffffull / kBpjBlock;
 static_assert(kBpjBlock == 1024, "the plan sort keys hold a 10-bit pass position");
 constexpr size_t kPlanQueueHeader = 256;
 // Default pixels per row of the ray tiles traced by the plan build: one image
@@ -589,6 +591,9 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
   }
   off[g.nbricks] = tot;
   cp[g.nbricks] = ch;
+  // one 1024-lane workgroup per chunk, and a dispatch holds 2^32 - 1 work-items
+  if (ch > kMaxChunksPerLaunch)
+    return set_error(TG_EINVAL, "plan of %llu chunks exceeds one launch; raise chunk_records", ch);
   // per-brick multiplier coprime with the brick's record count (bijective shuffle)
   std::vector<uint32_t> mult(g.nbricks, 1u);
   if (g.shuffle) {
@@ -765,7 +770,7 @@ int plan_bpj_batch(const tg_call *calls, const tg_bpj_plan *plans, int njobs, vo
                       "plan batch table copy")))
     return rc;
   if (total > 0) {
-    if (total > 0x7fffffffull) return set_error(TG_EINVAL, "plan batch has too many chunks");
+    if (total > kMaxChunksPerLaunch) return set_error(TG_EINVAL, "plan batch has too many chunks");
     static bool attr_set = false;
     if ((rc = allow_full_lds(k_plan_bpj_batch<T>, attr_set))) return rc;
     hipLaunchKernelGGL((k_plan_bpj_batch<T>), dim3((uint32_t)total), dim3(kBpjBlock), lds, s,

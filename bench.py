@@ -419,6 +419,16 @@ def cpu_baseline(ch, hs, cube_np, data_np, target_s):
            "kind": "port",
            "sample": "oracle pj+bpj on views 0..%d of the workload (%d of %d), %.1f s" %
                      (k2 - 1, k2, nv, dt)}
+    # one thread, as the reference runs with nthread=1: two views
+    k1 = min(2, nv)
+    d1 = np.zeros_like(data_np)
+    c1 = np.zeros_like(cube_np)
+    t0 = time.perf_counter()
+    s1 = O.siddon(O.PJ, recs, mrec, cube_np, d1, t0=0, t1=k1, nthread=1)
+    s2 = O.siddon(O.BPJ, recs, mrec, c1, data_np, t0=0, t1=k1, nthread=1)
+    dt1 = time.perf_counter() - t0
+    cpu["single_thread"] = {"value": (s1 + s2) / dt1, "cores": 1,
+                            "sample": "oracle pj+bpj on views 0..%d, %.1f s" % (k1 - 1, dt1)}
     return cpu, d[..., :k2], c, k2
 
 

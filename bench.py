@@ -258,10 +258,26 @@ def batch_leg():
     ms_b = _event_ms(batched, 3)
     ms_1 = _event_ms(one_by_one, 3)
     steps = sum(P.count_steps() for P in ops)
+    # one batched pair from zero against the same jobs run one by one in float64
+    for o, c in zip(outs, cubes):
+        o.zero_()
+        c.zero_()
+    batched()
+    ops64, xs64 = W.config4_problem(np.float64)
+    err_f, err_b = [], []
+    for P64, x64, o, c in zip(ops64, xs64, outs, cubes):
+        o64 = P64.matvec(x64)
+        err_f.append(_rel(o, o64))
+        err_b.append(_rel(c, P64.rmatvec(o64)))
+    del ops64, xs64
     return {"workload": "64 jobs, cubes 32^3-192^3, 8-96 views of 64^2-512^2, half Thomson, fp32",
             "jobs": len(ops), "ms_batched_pair": ms_b, "ms_one_by_one_pair": ms_1,
             "speedup": ms_1 / ms_b, "siddon_steps_per_pair": 2 * steps,
-            "steps_per_s_batched": 2 * steps / (ms_b * 1e-3)}
+            "steps_per_s_batched": 2 * steps / (ms_b * 1e-3),
+            "error_vs_fp64": {"forward_rel_l2_max": max(err_f),
+                              "forward_rel_l2_median": float(np.median(err_f)),
+                              "backward_rel_l2_max": max(err_b),
+                              "backward_rel_l2_median": float(np.median(err_b))}}
 
 
 def serving_leg(args, small=False):
@@ -279,14 +295,15 @@ def serving_leg(args, small=False):
     n_req = 2 * args.serve_requests if small else args.serve_requests
     slots = 32 if small else args.serve_max_active
 
-    def requests():
+    def requests(dtype=np.float32):
         rng = np.random.default_rng(W.SEED0 + (6 if small else 5))
         if small:
             mix = W.config5_mix(n_req, rng, n_range=(32, 64), v_range=(8, 24), pix=(64, 128),
                                 max_bins=4)
         else:
             mix = W.config5_mix(n_req, rng)
-        return SV.make_requests(mix, rng, rate=args.serve_rate or None, tol=1e-4, maxiter=200)
+        return SV.make_requests(mix, rng, rate=args.serve_rate or None, tol=1e-4, maxiter=200,
+                                dtype=dtype)
 
     plain_reqs = requests()
     served_reqs = requests()
@@ -306,12 +323,22 @@ def serving_leg(args, small=False):
     print("serving: served done in %.1f s" % served["makespan_s"], file=sys.stderr, flush=True)
     err = max(float(np.linalg.norm(a.x - b.x) / max(np.linalg.norm(b.x), 1e-300))
               for a, b in zip(served_reqs, plain_reqs))
+    # the served float32 solutions of the 8 lightest requests against float64
+    # solves of the same requests
+    light = sorted(range(len(served_reqs)), key=lambda k: served_reqs[k].work())[:8]
+    reqs64 = requests(np.float64)
+    SV.serve_plain([reqs64[k] for k in light])
+    err64 = [float(np.linalg.norm(served_reqs[k].x - reqs64[k].x) / np.linalg.norm(reqs64[k].x))
+             for k in light]
+    del reqs64
     out = {"requests": len(plain_reqs), "arrival_rate_rps": args.serve_rate or None,
             "max_active": slots, "tol": 1e-4,
             "served": served, "plain": plain,
             "throughput_ratio": served["throughput_rps"] / plain["throughput_rps"],
             "p50_ratio": plain["p50_s"] / served["p50_s"],
             "max_rel_l2_served_vs_plain": err,
+            "error_vs_fp64": {"requests": len(light), "solution_rel_l2_max": max(err64),
+                              "solution_rel_l2_median": float(np.median(err64))},
             "same_iterations": all(a.iterations == b.iterations
                                    for a, b in zip(served_reqs, plain_reqs)),
             "all_converged": all(r.residual <= 1e-4 for r in served_reqs + plain_reqs)}

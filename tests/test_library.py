@@ -36,6 +36,13 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert b"gfx950" in lib.tg_version()
 
 
+def test_integration_notes_list_every_declared_symbol():
+    """The integration notes name every entry point the header declares."""
+    notes = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    missing = [n for n in declared_functions() if n not in notes]
+    assert not missing, missing
+
+
 def test_missing_library_raises_instead_of_falling_back(monkeypatch):
     """Without the HIP library the package refuses to compute: no CPU path."""
     from tomograpy_amd import siddon as S

@@ -229,11 +229,14 @@ int tg_bpj_plan_batch(const tg_call *calls, const tg_bpj_plan *plans, int32_t nj
  * array of shape[0..3] (unused trailing dims = 1), with D_k the forward
  * difference along axis k in the numpy.diff convention (output shorter by
  * one), k selected by bit k of axes_mask.  Entries whose voxmask byte is
- * nonzero are left untouched (voxmask may be NULL).  Replaces the smoothness
- * priors lo.diff of models.py:85-94 inside the CG normal equations. */
+ * nonzero are left untouched (voxmask may be NULL).  With a per-voxel weight
+ * w (element type of dtype, same shape; NULL = none) the prior is the height
+ * prior D_k diag(w) and the term sum_k diag(w) D_k^T D_k diag(w) x.
+ * Replaces the smoothness priors lo.diff of models.py:85-104 (height_prior:
+ * w = r^2 from _radius_map) inside the CG normal equations. */
 int tg_regularize(int32_t dtype, const int32_t shape[4], uint32_t axes_mask, double lam,
                   int32_t identity, const void *x, void *out, const uint8_t *voxmask,
-                  void *stream);
+                  const void *weight, void *stream);
 
 /* Fused CG vector kernels over contiguous length-n vectors (the lo.acg
  * consumer of the projector pair, srt.py:225).  `scal` is a device array of
@@ -252,16 +255,20 @@ int tg_cg_update(int32_t dtype, int64_t n, void *x, void *r, const void *p, cons
                  double *scal, void *stream);
 int tg_cg_direction(int32_t dtype, int64_t n, const void *r, void *p, double *scal, void *stream);
 
-/* The smoothness priors of the regularised models (lo.diff, models.py:85-94,
+/* The smoothness priors of the regularised models (lo.diff, models.py:85-104,
  * and the temporal term of stsrt, models.py:140) in differentiable form, over
  * a C-contiguous array of shape[4] (trailing 1s for fewer axes):
- *   scal[slot] = sum_k weights[k] * sum_m (x[m + e_k] - x[m])^2
+ *   scal[slot] = sum_k weights[k] * sum_m (u[m + e_k] - u[m])^2
  * (the numpy.diff convention; squares and sums in double, in a fixed order)
  * and, when grad is not NULL,
- *   grad = 2 sum_k weights[k] D_k^T D_k x.
- * Axes with weights[k] == 0 are skipped; scal as for tg_cg_dot. */
+ *   grad = 2 diag(v) sum_k weights[k] D_k^T D_k u,
+ * where u = v x with the per-voxel weight v = voxel_weight (the height prior
+ * r^2 of models.py:90-93; same shape and element type as x), or u = x and
+ * v = 1 when voxel_weight is NULL.  Axes with weights[k] == 0 are skipped;
+ * scal as for tg_cg_dot. */
 int tg_smoothness(int32_t dtype, const int32_t shape[4], const double weights[4], const void *x,
-                  void *grad, double *scal, int32_t slot, void *stream);
+                  const void *voxel_weight, void *grad, double *scal, int32_t slot,
+                  void *stream);
 
 /* ---- segmented CG: the vector work of many concurrent solves in one launch
  * (no reference counterpart; used by the reconstruction server) ---- */
@@ -270,6 +277,7 @@ int tg_smoothness(int32_t dtype, const int32_t shape[4], const double weights[4]
 typedef struct tg_cg_seg {
   void *x, *r, *p, *q;
   const uint8_t *voxmask; /* may be NULL */
+  const void *weight;     /* per-voxel weight of the height prior (see tg_regularize), or NULL */
   double *scal;           /* this solve's CG scalars, layout of tg_cg_update */
   int64_t n;
   int64_t block0;         /* first launch block of this segment: sum of tg_cg_seg_blocks(n) before it */
@@ -279,7 +287,8 @@ typedef struct tg_cg_seg {
   double lam;
 } tg_cg_seg;
 
-#define TG_CG_OP_REGULARIZE 0 /* q += lam ((identity ? p : 0) + sum_k D_k^T D_k p) */
+#define TG_CG_OP_REGULARIZE 0 /* q += lam ((identity ? p : 0) + sum_k D_k^T D_k p), weighted as
+                                 tg_regularize when tg_cg_seg.weight is set */
 #define TG_CG_OP_DOT_PQ 1     /* scal[1] = p.q */
 #define TG_CG_OP_UPDATE 2     /* alpha = scal[0]/scal[1]; x += alpha p; r -= alpha q; scal[2] = r.r */
 #define TG_CG_OP_DIRECTION 3  /* beta = scal[2]/scal[0]; p = r + beta p; scal[0] = scal[2] */

@@ -81,7 +81,7 @@ def test_regularizer_matches_numpy():
         bits = sum(1 << a for a in axes)
         shp = (ctypes.c_int32 * 4)(*shape)
         N.check(N.lib().tg_regularize(N.TG_F64, shp, bits, 0.3, 1, xd.data_ptr(), od.data_ptr(),
-                                      None, S.stream_handle()))
+                                      None, None, S.stream_handle()))
         ref = out0 + regularizer_reference(x, axes, 0.3, True)
         np.testing.assert_allclose(od.cpu().numpy(), ref, rtol=1e-13, atol=1e-13)
 

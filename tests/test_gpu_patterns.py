@@ -155,3 +155,83 @@ def test_plan_batch_mixed_jobs_match_one_job_calls():
             assert np.all(np.abs(a - b)[d] <= np.spacing(np.maximum(np.abs(a), np.abs(b)))[d])
         else:
             assert rel_l2(a, b) <= 1e-5
+
+
+@pytest.mark.parametrize("mode", ["plan", "atomic"])
+def test_batched_backward_repeated_operator(mode):
+    """One operator listed twice in a backward batch: each job sums in its own
+    float64 accumulator, so B.run([P, P], [y1, y2]) equals the two rmatvec
+    calls (to one-ulp ties), for brick-plan and atomic jobs alike."""
+    print("repeated operator", mode, flush=True)
+    ch = W.cube_header(40)
+    hs = W.view_headers(48, np.linspace(0.1, 5.0, 9), d=200.0)
+    P = Siddon(hs, ch, dtype=np.float32, bpj_mode=mode)
+    assert P.acc64
+    g = torch.Generator().manual_seed(9)
+    y1 = torch.rand((48, 48, 9), generator=g).cuda()
+    y2 = torch.rand((48, 48, 9), generator=g).cuda()
+    r1 = P.rmatvec(y1).clone()
+    r2 = P.rmatvec(y2).clone()
+    q1, q2 = torch.zeros_like(r1), torch.zeros_like(r2)
+    B.run([P, P], [y1, y2], [q1, q2], backward=True)
+    torch.cuda.synchronize()
+    for s, o in ((r1, q1), (r2, q2)):
+        a, b = o.cpu().numpy(), s.cpu().numpy()
+        d = a != b
+        assert d.mean() <= 1e-3
+        assert np.all(np.abs(a - b)[d] <= np.spacing(np.maximum(np.abs(a), np.abs(b)))[d])
+    with pytest.raises(ValueError, match="twice"):
+        B.run([P, P], [y1, y2], [q1, q1], backward=True)
+
+
+def test_served_poisson_arrivals_match_plain():
+    """Poisson arrivals at a rate above what one request at a time can serve,
+    so that requests arrive while others are on the GPU and join the running
+    set: every result and iteration count equals the request's plain solve."""
+    print("served vs plain, Poisson arrivals", flush=True)
+    reqs = _requests(16, seed=7, rate=400.0)
+    plain = _requests(16, seed=7, rate=400.0)
+    assert reqs[-1].arrival > 0
+    server = SV.Server(budget_bytes=1 << 30, max_active=6, check_every=5)
+    s_sum = server.run(reqs)
+    p_sum = SV.serve_plain(plain, check_every=5, order="shortest")
+    assert s_sum["requests"] == p_sum["requests"] == 16
+    overlap = sum(1 for a in reqs for b in reqs
+                  if a is not b and b.arrival > 0 and a.t_admit < b.t_admit < a.t_done)
+    print("served", s_sum, "plain", p_sum, "admissions during another solve", overlap,
+          flush=True)
+    assert overlap > 0
+    for a, b in zip(reqs, plain):
+        assert a.status == b.status == "done"
+        assert a.residual <= 1e-4 and b.residual <= 1e-4
+        assert a.iterations == b.iterations, (a.id, a.iterations, b.iterations)
+        assert rel_l2(a.x, b.x) <= 1e-4, (a.id, rel_l2(a.x, b.x))
+
+
+def test_server_cancel_and_timeout():
+    """A request cancelled while on the GPU leaves the active set and frees its
+    slot; one whose timeout passes while it waits is never admitted; the others
+    complete with their plain results."""
+    print("cancel and timeout", flush=True)
+    reqs = _requests(6, seed=8)
+    plain = _requests(6, seed=8)
+    reqs[1].timeout = 0.0  # already past when the server first looks at it
+    victim = reqs[0]
+
+    def on_step(step, rs):
+        if step == 2:
+            victim.cancel()
+
+    # one slot, arrival order: request 0 runs first and is cancelled after two
+    # of its iterations (a check every 5 cannot have ended it)
+    server = SV.Server(budget_bytes=1 << 30, max_active=1, check_every=5, order="arrival")
+    out = server.run(reqs, on_step=on_step)
+    assert reqs[1].status == "timed out" and reqs[1].t_admit is None
+    assert victim.status == "cancelled" and victim.iterations == 2 and victim.x is None
+    assert out["dropped"] == 2 and out["requests"] == 4
+    assert sorted(server.free_rows) == [0]  # the cancelled request's slot came back
+    SV.serve_plain(plain, check_every=5)
+    for a, b in zip(reqs, plain):
+        if a.status == "done":
+            assert a.iterations == b.iterations
+            assert rel_l2(a.x, b.x) <= 1e-4

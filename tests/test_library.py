@@ -274,12 +274,50 @@ def test_smoothness_rejects_bad_arguments_without_gpu():
     lib = N.lib()
     shape = (ctypes.c_int32 * 4)(4, 4, 1, 1)
     w = (ctypes.c_double * 4)(1.0, 1.0, 0.0, 0.0)
-    assert lib.tg_smoothness(N.TG_F32, shape, w, None, None, None, 0, None) == N.TG_EINVAL
+    assert lib.tg_smoothness(N.TG_F32, shape, w, None, None, None, None, 0, None) == N.TG_EINVAL
     assert b"tg_smoothness" in lib.tg_last_error()
     bad = (ctypes.c_int32 * 4)(4, 0, 1, 1)
     x = np.zeros(16, np.float32)
     scal = np.zeros(N.TG_CG_SCAL, np.float64)
-    assert lib.tg_smoothness(N.TG_F32, bad, w, x.ctypes.data, None, scal.ctypes.data, 0,
+    assert lib.tg_smoothness(N.TG_F32, bad, w, x.ctypes.data, None, None, scal.ctypes.data, 0,
                              None) == N.TG_EINVAL
-    assert lib.tg_smoothness(N.TG_F32, shape, w, x.ctypes.data, None, scal.ctypes.data, 16,
+    assert lib.tg_smoothness(N.TG_F32, shape, w, x.ctypes.data, None, None, scal.ctypes.data, 16,
                              None) == N.TG_EINVAL
+
+
+@pytest.mark.parametrize("share", ["acc", "cube", "overlap", "none"])
+def test_batch_rejects_shared_accumulators(share):
+    """A backward batch's float64 accumulators are finished by one launch, each
+    job reading and clearing its own and adding it to its cube without atomics:
+    tg_batch_plan_table (host only) rejects two jobs whose accumulators, or whose
+    cubes, overlap."""
+    calls = (N.TgCall * 3)()
+    for k in range(3):
+        c = calls[k]
+        c.dtype = N.TG_F32
+        c.kind = N.TG_BPJ
+        c.flags = N.TG_ACC_F64
+        c.map.n[0] = c.map.n[1] = c.map.n[2] = 10
+        c.map.nt = 1
+        c.map.stride[0], c.map.stride[1], c.map.stride[2] = 100, 10, 1
+        c.n1 = c.n2 = 16
+        c.nv = 2
+        c.data_stride[0], c.data_stride[1], c.data_stride[2] = 16, 1, 256
+        c.views = 0x1000 + k
+        c.cube = 0x100000 * (k + 1)
+        c.data = 0x900000 * (k + 1)
+        c.acc = 0x4000000 * (k + 1)
+    if share == "acc":
+        calls[2].acc = calls[0].acc
+    elif share == "cube":
+        calls[1].cube = calls[0].cube
+    elif share == "overlap":
+        calls[1].acc = calls[0].acc + 8 * 999  # the last double of job 0's range
+    stage = np.zeros(N.lib().tg_batch_queue_bytes(3), dtype=np.uint8)
+    plan = N.TgBatchPlan()
+    rc = N.lib().tg_batch_plan_table(calls, 3, ctypes.byref(plan), stage.ctypes.data)
+    if share == "none":
+        assert rc == N.TG_OK
+    else:
+        assert rc == N.TG_EINVAL
+        assert b"share" in N.lib().tg_last_error() or b"same cube" in N.lib().tg_last_error()

@@ -48,7 +48,8 @@ class TgCall(ctypes.Structure):
 
 
 class TgCgSeg(ctypes.Structure):
-    _fields_ = [("x", _vp), ("r", _vp), ("p", _vp), ("q", _vp), ("voxmask", _vp), ("scal", _vp),
+    _fields_ = [("x", _vp), ("r", _vp), ("p", _vp), ("q", _vp), ("voxmask", _vp), ("weight", _vp),
+                ("scal", _vp),
                 ("n", _i64), ("block0", _i64), ("shape", _i32 * 4), ("axes", _u32),
                 ("identity", _i32), ("lam", _dbl)]
 
@@ -84,12 +85,13 @@ SIGNATURES = [
     ("tg_det_workspace_min_bytes", _sz, [ctypes.POINTER(TgCall)]),
     ("tg_scale_cube", ctypes.c_int, [ctypes.POINTER(TgCall), _vp]),
     ("tg_vox_apply", ctypes.c_int, [ctypes.POINTER(TgCall), _vp, ctypes.POINTER(_i64), _vp]),
-    ("tg_regularize", ctypes.c_int, [_i32, ctypes.POINTER(_i32), _u32, _dbl, _i32, _vp, _vp, _vp, _vp]),
+    ("tg_regularize", ctypes.c_int, [_i32, ctypes.POINTER(_i32), _u32, _dbl, _i32, _vp, _vp, _vp,
+                                     _vp, _vp]),
     ("tg_cg_dot", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _i32, _vp]),
     ("tg_cg_update", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("tg_cg_direction", ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp]),
-    ("tg_smoothness", ctypes.c_int, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_dbl), _vp, _vp, _vp,
-                                     _i32, _vp]),
+    ("tg_smoothness", ctypes.c_int, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_dbl), _vp, _vp,
+                                     _vp, _vp, _i32, _vp]),
     ("tg_bpj_plan_init", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan)]),
     ("tg_bpj_plan_count", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan), _vp]),
     ("tg_bpj_plan_build", ctypes.c_int, [ctypes.POINTER(TgCall), ctypes.POINTER(TgBpjPlan), _vp]),

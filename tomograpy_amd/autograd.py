@@ -69,10 +69,12 @@ def backproject(y, P):
 
 class Smoothness(torch.autograd.Function):
     """sum_k w_k ||D_k x||^2 (D_k: numpy.diff along axis k) with the gradient
-    2 sum_k w_k D_k^T D_k x, both from one pass of ``tg_smoothness``."""
+    2 sum_k w_k D_k^T D_k x, both from one pass of ``tg_smoothness``.  With a
+    per-voxel weight v (the height prior r^2, models.py:90-93) the differences
+    are those of v x and the gradient is 2 v sum_k w_k D_k^T D_k (v x)."""
 
     @staticmethod
-    def forward(ctx, x, weights):
+    def forward(ctx, x, weights, voxel_weight=None):
         if x.dim() > 4:
             raise ValueError("smoothness takes arrays of up to 4 axes")
         if x.dtype not in (torch.float32, torch.float64):
@@ -81,12 +83,19 @@ class Smoothness(torch.autograd.Function):
         if len(w) != x.dim():
             raise ValueError("one weight per axis: %d weights for %d axes" % (len(w), x.dim()))
         xc = x.detach().contiguous()
+        vw = None
+        if voxel_weight is not None:
+            vw = torch.as_tensor(voxel_weight).to(xc.device, xc.dtype).contiguous()
+            if tuple(vw.shape) != tuple(xc.shape):
+                raise ValueError("voxel weight of shape %s for an array of shape %s"
+                                 % (tuple(vw.shape), tuple(xc.shape)))
         shape = (ctypes.c_int32 * 4)(*(list(xc.shape) + [1] * (4 - xc.dim())))
         wts = (ctypes.c_double * 4)(*(w + [0.0] * (4 - len(w))))
         grad = torch.empty_like(xc)
         scal = torch.empty(N.TG_CG_SCAL, dtype=torch.float64, device=xc.device)
         code = N.TG_F32 if xc.dtype == torch.float32 else N.TG_F64
         N.check(N.lib().tg_smoothness(code, shape, wts, ctypes.c_void_p(xc.data_ptr()),
+                                      ctypes.c_void_p(vw.data_ptr() if vw is not None else None),
                                       ctypes.c_void_p(grad.data_ptr()),
                                       ctypes.c_void_p(scal.data_ptr()), 0, S.stream_handle()))
         ctx.save_for_backward(grad)
@@ -95,9 +104,10 @@ class Smoothness(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gout):
         (grad,) = ctx.saved_tensors
-        return grad * gout, None
+        return grad * gout, None, None
 
 
-def smoothness(x, weights):
-    """Differentiable sum_k weights[k] ||D_k x||^2 over the axes of x."""
-    return Smoothness.apply(x, tuple(weights))
+def smoothness(x, weights, voxel_weight=None):
+    """Differentiable sum_k weights[k] ||D_k (v x)||^2 over the axes of x, with
+    v = voxel_weight (the height prior) or 1."""
+    return Smoothness.apply(x, tuple(weights), voxel_weight)

@@ -111,7 +111,12 @@ class Launch:
         self.alive = []      # every tensor the packed calls point to
         calls = (N.TgCall * n)()
         weighted = set()
+        accs = set()
+        targets = set()
         for k, (P, x, y) in enumerate(zip(ops, ins, outs)):
+            if y.data_ptr() in targets:
+                raise ValueError("job %d: a batch cannot list one output twice" % k)
+            targets.add(y.data_ptr())
             if self.backward:
                 target = y
                 if P.vox_weight is not None:
@@ -119,6 +124,14 @@ class Launch:
                     self.post.append((P, target, y))
                 data = self._operand(P._as_stack(x), x)
                 c = P._call(N.TG_BPJ, target, data)
+                if c.flags & N.TG_ACC_F64:
+                    if c.acc in accs:
+                        # the operator's float64 accumulator serves its first job;
+                        # a repeat of the operator sums into a buffer of its own
+                        own = torch.zeros_like(P.acc_buffer())
+                        self.alive.append(own)
+                        c.acc = own.data_ptr()
+                    accs.add(c.acc)
             else:
                 target = y
                 if P.pix_weight is not None and y.stride() != P.pix_weight.stride():

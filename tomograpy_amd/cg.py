@@ -35,7 +35,7 @@ class CGSolve:
     """
 
     def __init__(self, P, y, lam=1e-2, identity=True, axes=None, tol=1e-4, maxiter=50,
-                 scal=None):
+                 scal=None, weight=None):
         self.P = P
         self.lam = float(lam)
         self.identity = bool(identity)
@@ -63,6 +63,7 @@ class CGSolve:
         vm = getattr(P, "voxmask", None)
         if vm is not None:
             self.mask = vm.expand(self.store_shape).contiguous() if self.time else vm
+        self.weight = _prior_weight(P, axes, weight, self.store_shape)
         self.y = P._as_stack(y)
         self.out = P.xout  # forward output buffer of this solve
         # r0 = P^T y (x0 = 0), p0 = r0
@@ -82,6 +83,8 @@ class CGSolve:
         g.x, g.r, g.p, g.q = (t.data_ptr() for t in (self.xs, self.rs, self.ps, self.qs))
         if self.mask is not None:
             g.voxmask = self.mask.data_ptr()
+        if self.weight is not None:
+            g.weight = self.weight.data_ptr()
         g.scal = self.scal.data_ptr()
         g.n = self.n
         g.block0 = int(block0)
@@ -99,6 +102,25 @@ class CGSolve:
     def residual(self, scal_host=None):
         v = self.scal.cpu().numpy() if scal_host is None else scal_host
         return float(np.sqrt(v[0] / v[5])) if v[5] > 0 else 0.0
+
+
+def _prior_weight(P, axes, weight, shape):
+    """The height prior's per-voxel weight on the device (operator dtype,
+    `shape`), from `weight` or from a SmoothnessPrior passed as `axes`; None
+    without one.  Time-binned maps have none (the reference's _radius_map,
+    models.py:96-104, unpacks three axes and fails on a 4-D map)."""
+    if weight is None:
+        weight = getattr(axes, "weight", None)
+    if weight is None:
+        return None
+    if isinstance(P, Siddon4d):
+        raise NotImplementedError("the height prior is defined for 3-D maps only")
+    w = weight if isinstance(weight, torch.Tensor) else torch.from_numpy(np.asarray(weight))
+    w = w.to(P.dev, P.tdtype).contiguous()
+    if tuple(w.shape) != tuple(shape):
+        raise ValueError("prior weight of shape %s for a map of shape %s"
+                         % (tuple(w.shape), tuple(shape)))
+    return w
 
 
 class SegTable:
@@ -171,13 +193,18 @@ def iterate(solves, table, tuner=None):
 
 
 def solve(P, y, lam=1e-2, identity=True, axes=None, maxiter=50, tol=None, check_every=10,
-          callback=None):
+          callback=None, weight=None):
     """Solves the regularised normal equations for data y (device, P.shapeout).
+
+    ``axes`` are the smoothness priors' axes (default: every axis), or a
+    ``models.SmoothnessPrior``, which also carries the per-voxel weight of the
+    height prior; ``weight`` gives that weight explicitly.
 
     Returns (x, info): x in the projector's layout, info = {"iterations",
     "residual" (||r|| / ||r0||)}.
     """
-    s = CGSolve(P, y, lam=lam, identity=identity, axes=axes, tol=tol, maxiter=maxiter)
+    s = CGSolve(P, y, lam=lam, identity=identity, axes=axes, tol=tol, maxiter=maxiter,
+                weight=weight)
     table = SegTable([s], P.dev)
     table.op(N.TG_CG_OP_INIT)
     for it in range(1, maxiter + 1):
@@ -197,12 +224,13 @@ class NormalOperator:
     """x -> P^T P x + lam ((x if identity) + sum_k D_k^T D_k x): the operator
     ``solve`` applies, for use outside a solve (benchmarks, checks)."""
 
-    def __init__(self, P, lam=1e-2, identity=True, axes=None):
+    def __init__(self, P, lam=1e-2, identity=True, axes=None, weight=None):
         self.P = P
         self.lam = float(lam)
         self.identity = bool(identity)
         nd = len(P.shapein)
         self.axes = tuple(range(nd)) if axes is None else tuple(axes)
+        self.weight = _prior_weight(P, axes, weight, tuple(P.shapein))
         bits = 0
         for a in self.axes:
             bits |= 1 << a
@@ -219,7 +247,8 @@ class NormalOperator:
         P.backproject_into(P.xout, out)
         if self.lam != 0.0:
             vm = P.voxmask.data_ptr() if getattr(P, "voxmask", None) is not None else None
+            w = self.weight.data_ptr() if self.weight is not None else None
             N.check(N.lib().tg_regularize(P.code, self.shape4, self.mask_bits, self.lam,
                                           int(self.identity), x.data_ptr(), out.data_ptr(), vm,
-                                          S.stream_handle()))
+                                          w, S.stream_handle()))
         return out

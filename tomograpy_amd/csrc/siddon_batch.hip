@@ -134,6 +134,12 @@ int run_batch(const tg_call *calls, int njobs, const tg_batch_plan *plan, void *
     // longest rays first: a ray crosses at most n0 + n1 + n2 voxels
     cost[k] = (double)(cc.map.n[0] + cc.map.n[1] + cc.map.n[2]);
   }
+  if (bwd) {
+    std::vector<const KArgs<T> *> pa(njobs);
+    for (int k = 0; k < njobs; k++) pa[k] = &args[k];
+    int rc = check_acc_jobs<T>(pa.data(), njobs);
+    if (rc) return rc;
+  }
   // variant of each job: bit 0 obstacle, bit 1 voxel operand, bit 2 32-bit walk
   std::vector<int> variant(njobs);
   for (int k = 0; k < njobs; k++) {

@@ -5,6 +5,9 @@
 // D_k is the forward difference along axis k with the numpy.diff shape
 // convention (output shorter by one), so D_k^T D_k x at index m along k is
 //   [m >= 1] (x_m - x_{m-1}) - [m <= n-2] (x_{m+1} - x_m).
+// With a per-voxel weight w (the height prior of models.py:85-104, where each
+// D_k is composed with diag(r^2)), the prior term is ||D_k diag(w) x||^2 and
+// its normal-equation operator diag(w) D_k^T D_k diag(w) x.
 // Dot products are reduced in a fixed order (per-block partials, then one
 // block), so a CG run is reproducible given reproducible operator outputs.
 #include <hip/hip_runtime.h>
@@ -18,41 +21,63 @@ namespace {
 constexpr int kRedBlocks = 1024;  // max partials, lives at scal[16 .. 16 + kRedBlocks)
 constexpr int kScalPartials = 16;
 
+// (identity ? x_k : 0) + sum_k D_k^T D_k x at flat index k of an array of
+// shape n[4] (C order), or with the voxel weight w (not null)
+// (identity ? x_k : 0) + w_k sum_k D_k^T D_k (w x).
 template <typename T>
-__global__ __launch_bounds__(256) void k_regularize(int64_t n0, int64_t n1, int64_t n2,
-                                                    int64_t n3, uint32_t axes, T lam,
-                                                    int identity, const T *__restrict__ x,
-                                                    T *__restrict__ out,
-                                                    const uint8_t *__restrict__ mask) {
-  const int64_t total = n0 * n1 * n2 * n3;
-  const int64_t s3 = 1, s2 = n3, s1 = n2 * n3, s0 = n1 * n2 * n3;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    if (mask && mask[k]) continue;
-    const int64_t m3 = k % n3, m2 = (k / s2) % n2, m1 = (k / s1) % n1, m0 = k / s0;
-    const T xc = x[k];
-    T acc = identity ? xc : (T)0;
-    const int64_t m[4] = {m0, m1, m2, m3};
-    const int64_t n[4] = {n0, n1, n2, n3};
-    const int64_t st[4] = {s0, s1, s2, s3};
+__device__ __forceinline__ T reg_at(const int64_t n[4], uint32_t axes, int identity,
+                                    const T *__restrict__ x, const T *__restrict__ w, int64_t k) {
+  const int64_t st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
+  const int64_t m[4] = {k / st[0], (k / st[1]) % n[1], (k / st[2]) % n[2], k % n[3]};
+  const T xc = x[k];
+  T acc = identity ? xc : (T)0;
+  if (!w) {
 #pragma unroll
     for (int ax = 0; ax < 4; ax++) {
       if (!(axes & (1u << ax))) continue;
       if (m[ax] >= 1) acc += xc - x[k - st[ax]];
       if (m[ax] <= n[ax] - 2) acc -= x[k + st[ax]] - xc;
     }
-    out[k] = out[k] + lam * acc;
+    return acc;
+  }
+  const T wc = w[k];
+  const T uc = wc * xc;
+  T dd = (T)0;
+#pragma unroll
+  for (int ax = 0; ax < 4; ax++) {
+    if (!(axes & (1u << ax))) continue;
+    if (m[ax] >= 1) dd += uc - w[k - st[ax]] * x[k - st[ax]];
+    if (m[ax] <= n[ax] - 2) dd -= w[k + st[ax]] * x[k + st[ax]] - uc;
+  }
+  return acc + wc * dd;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_regularize(int64_t n0, int64_t n1, int64_t n2,
+                                                    int64_t n3, uint32_t axes, T lam,
+                                                    int identity, const T *__restrict__ x,
+                                                    T *__restrict__ out,
+                                                    const uint8_t *__restrict__ mask,
+                                                    const T *__restrict__ w) {
+  const int64_t total = n0 * n1 * n2 * n3;
+  const int64_t n[4] = {n0, n1, n2, n3};
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    if (mask && mask[k]) continue;
+    out[k] = out[k] + lam * reg_at<T>(n, axes, identity, x, w, k);
   }
 }
 
 // Smoothness prior sum_k w_k ||D_k x||^2 and its gradient 2 sum_k w_k D_k^T D_k x
 // in one pass: per-block partial sums of the value (double) at
 // scal[kScalPartials + blockIdx.x], the gradient (in T) at grad (if not null).
+// With a voxel weight vw (not null) the differences are those of u = vw x and
+// the gradient is 2 vw sum_k w_k D_k^T D_k u.
 template <typename T>
 __global__ __launch_bounds__(256) void k_smooth(int64_t n0, int64_t n1, int64_t n2, int64_t n3,
                                                 double w0, double w1, double w2, double w3,
                                                 const T *__restrict__ x, T *__restrict__ grad,
-                                                double *scal) {
+                                                double *scal, const T *__restrict__ vw) {
   __shared__ double lds[256];
   const int64_t total = n0 * n1 * n2 * n3;
   const int64_t s3 = 1, s2 = n3, s1 = n2 * n3, s0 = n1 * n2 * n3;
@@ -63,20 +88,24 @@ __global__ __launch_bounds__(256) void k_smooth(int64_t n0, int64_t n1, int64_t 
     const int64_t m[4] = {k / s0, (k / s1) % n1, (k / s2) % n2, k % n3};
     const int64_t n[4] = {n0, n1, n2, n3};
     const int64_t st[4] = {s0, s1, s2, s3};
-    const T xc = x[k];
+    const T xc = vw ? vw[k] * x[k] : x[k];
     T g = (T)0;
 #pragma unroll
     for (int ax = 0; ax < 4; ax++) {
       if (w[ax] == 0.0) continue;
       const T wt = (T)w[ax];
       if (m[ax] <= n[ax] - 2) {  // the difference x[m + e] - x[m] belongs to m
-        const T d = x[k + st[ax]] - xc;
+        const int64_t kn = k + st[ax];
+        const T d = (vw ? vw[kn] * x[kn] : x[kn]) - xc;
         v += w[ax] * ((double)d * (double)d);
         g -= wt * d;
       }
-      if (m[ax] >= 1) g += wt * (xc - x[k - st[ax]]);
+      if (m[ax] >= 1) {
+        const int64_t kp = k - st[ax];
+        g += wt * (xc - (vw ? vw[kp] * x[kp] : x[kp]));
+      }
     }
-    if (grad) grad[k] = (T)2 * g;
+    if (grad) grad[k] = vw ? (T)2 * (vw[k] * g) : (T)2 * g;
   }
   lds[threadIdx.x] = v;
   __syncthreads();
@@ -178,20 +207,8 @@ __device__ __forceinline__ int find_segment(const tg_cg_seg *segs, int nseg, int
 
 template <typename T>
 __device__ __forceinline__ T reg_value(const tg_cg_seg &g, const T *x, int64_t k) {
-  const int64_t n0 = g.shape[0], n1 = g.shape[1], n2 = g.shape[2], n3 = g.shape[3];
-  const int64_t s3 = 1, s2 = n3, s1 = n2 * n3, s0 = n1 * n2 * n3;
-  const int64_t m[4] = {k / s0, (k / s1) % n1, (k / s2) % n2, k % n3};
-  const int64_t n[4] = {n0, n1, n2, n3};
-  const int64_t st[4] = {s0, s1, s2, s3};
-  const T xc = x[k];
-  T acc = g.identity ? xc : (T)0;
-#pragma unroll
-  for (int ax = 0; ax < 4; ax++) {
-    if (!(g.axes & (1u << ax))) continue;
-    if (m[ax] >= 1) acc += xc - x[k - st[ax]];
-    if (m[ax] <= n[ax] - 2) acc -= x[k + st[ax]] - xc;
-  }
-  return acc;
+  const int64_t n[4] = {g.shape[0], g.shape[1], g.shape[2], g.shape[3]};
+  return reg_at<T>(n, g.axes, g.identity, x, (const T *)g.weight, k);
 }
 
 template <typename T, int OP>
@@ -324,7 +341,7 @@ extern "C" {
 
 int tg_regularize(int32_t dtype, const int32_t shape[4], uint32_t axes_mask, double lam,
                   int32_t identity, const void *x, void *out, const uint8_t *voxmask,
-                  void *stream) {
+                  const void *weight, void *stream) {
   if (!shape || !x || !out) return set_error(TG_EINVAL, "null argument to tg_regularize");
   for (int k = 0; k < 4; k++)
     if (shape[k] <= 0) return set_error(TG_EINVAL, "shape[%d] = %d must be positive", k, shape[k]);
@@ -333,18 +350,19 @@ int tg_regularize(int32_t dtype, const int32_t shape[4], uint32_t axes_mask, dou
   if (dtype == TG_F32)
     hipLaunchKernelGGL((k_regularize<float>), dim3(ew_blocks(n)), dim3(256), 0, s, shape[0],
                        shape[1], shape[2], shape[3], axes_mask, (float)lam, identity,
-                       (const float *)x, (float *)out, voxmask);
+                       (const float *)x, (float *)out, voxmask, (const float *)weight);
   else if (dtype == TG_F64)
     hipLaunchKernelGGL((k_regularize<double>), dim3(ew_blocks(n)), dim3(256), 0, s, shape[0],
                        shape[1], shape[2], shape[3], axes_mask, lam, identity, (const double *)x,
-                       (double *)out, voxmask);
+                       (double *)out, voxmask, (const double *)weight);
   else
     return set_error(TG_EINVAL, "bad dtype %d", dtype);
   return hip_check(hipGetLastError(), "regularize launch");
 }
 
 int tg_smoothness(int32_t dtype, const int32_t shape[4], const double weights[4], const void *x,
-                  void *grad, double *scal, int32_t slot, void *stream) {
+                  const void *voxel_weight, void *grad, double *scal, int32_t slot,
+                  void *stream) {
   if (!shape || !weights || !x || !scal || slot < 0 || slot >= 16)
     return set_error(TG_EINVAL, "bad tg_smoothness arguments");
   for (int k = 0; k < 4; k++)
@@ -355,11 +373,11 @@ int tg_smoothness(int32_t dtype, const int32_t shape[4], const double weights[4]
   if (dtype == TG_F32)
     hipLaunchKernelGGL((k_smooth<float>), dim3(nb), dim3(256), 0, s, shape[0], shape[1], shape[2],
                        shape[3], weights[0], weights[1], weights[2], weights[3], (const float *)x,
-                       (float *)grad, scal);
+                       (float *)grad, scal, (const float *)voxel_weight);
   else if (dtype == TG_F64)
     hipLaunchKernelGGL((k_smooth<double>), dim3(nb), dim3(256), 0, s, shape[0], shape[1],
                        shape[2], shape[3], weights[0], weights[1], weights[2], weights[3],
-                       (const double *)x, (double *)grad, scal);
+                       (const double *)x, (double *)grad, scal, (const double *)voxel_weight);
   else
     return set_error(TG_EINVAL, "bad dtype %d", dtype);
   hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s, scal, (int)nb, slot);

@@ -24,6 +24,34 @@ template <typename T>
 int finish_acc_table(const void *dev_table, size_t stride, const KArgs<T> *const *host_args,
                      int njobs, hipStream_t s);
 
+// The float64 accumulators of a batch's jobs are finished by one launch that
+// runs every job's finish at once, each reading and clearing its accumulator
+// and adding it to its cube without atomics.  Two jobs may therefore share
+// neither an accumulator nor a cube.  Returns TG_EINVAL (with the job indices)
+// when the ranges of two jobs' accumulators, or of their cubes, overlap.
+template <typename T>
+int check_acc_jobs(const KArgs<T> *const *args, int njobs) {
+  for (int x = 0; x < njobs; x++) {
+    const KArgs<T> &a = *args[x];
+    if (!a.acc) continue;
+    const int64_t na = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.nt;
+    const char *a0 = (const char *)a.acc, *a1 = a0 + na * (int64_t)sizeof(double);
+    const char *c0 = (const char *)a.cube, *c1 = c0 + na * (int64_t)sizeof(T);
+    for (int y = x + 1; y < njobs; y++) {
+      const KArgs<T> &b = *args[y];
+      if (!b.acc) continue;
+      const int64_t nb = (int64_t)b.n[0] * b.n[1] * b.n[2] * b.nt;
+      const char *b0 = (const char *)b.acc, *b1 = b0 + nb * (int64_t)sizeof(double);
+      const char *d0 = (const char *)b.cube, *d1 = d0 + nb * (int64_t)sizeof(T);
+      if (a0 < b1 && b0 < a1)
+        return set_error(TG_EINVAL, "jobs %d and %d share a float64 accumulator", x, y);
+      if (c0 < d1 && d0 < c1)
+        return set_error(TG_EINVAL, "jobs %d and %d accumulate into the same cube", x, y);
+    }
+  }
+  return TG_OK;
+}
+
 // Deterministic backprojection (siddon_det.hip).
 template <typename T>
 int run_deterministic(const tg_call *c, const KArgs<T> &a, hipStream_t s);

@@ -763,6 +763,12 @@ int plan_bpj_batch(const tg_call *calls, const tg_bpj_plan *plans, int njobs, vo
     lds = l > lds ? l : lds;
     memcpy(&table[k], &j, sizeof(j));
   }
+  {
+    std::vector<const KArgs<T> *> pa(njobs);
+    for (int k = 0; k < njobs; k++) pa[k] = &table[k].a;
+    int rc = check_acc_jobs<T>(pa.data(), njobs);
+    if (rc) return rc;
+  }
   if (!queue) return TG_OK;
   const size_t bytes = kPlanQueueHeader + (size_t)njobs * sizeof(PlanJob<T>);
   int rc;

@@ -319,22 +319,46 @@ def check_headers(data_header, cube_header):
     complete_view_headers(data_header)
     if "MMAX1" not in cube_header:
         map_borders(cube_header)
+    view_records(data_header)
+    map_record(cube_header)
     return data_header, cube_header
 
 
+MAP_KEYS = [p + str(i) for p in ("CDELT", "PSHAPE", "MMIN", "MMAX") for i in (1, 2, 3)]
+
+
+def _header_values(h, keys, what):
+    """The values of `keys` in header dict h as float64, checked before any
+    device work.  The reference reads each key with PyDict_GetItemString and
+    PyFloat_AsDouble (C_siddon.c.template:1051-1056), which fails obscurely on a
+    missing key, and a NaN in the geometry makes its voxel() cascade
+    (C:811-912) take no branch and loop forever.  Here a missing key raises
+    KeyError and a non-finite value ValueError, naming the key."""
+    out = np.empty(len(keys), dtype=np.float64)
+    for k, key in enumerate(keys):
+        if key not in h:
+            raise KeyError("%s header has no %s" % (what, key))
+        out[k] = float(h[key])
+    bad = ~np.isfinite(out)
+    if bad.any():
+        k = int(np.argmax(bad))
+        raise ValueError("%s header: %s = %r is not finite" % (what, keys[k], h[keys[k]]))
+    return out
+
+
 def view_records(data_header):
-    """(nv, 18) float64 array of the values the projector reads per view."""
+    """(nv, 18) float64 array of the values the projector reads per view
+    (C_siddon.c.template:1077-1101); raises on missing or non-finite values."""
     rec = np.empty((len(data_header), len(VIEW_KEYS)), dtype=np.float64)
     for t, h in enumerate(data_header):
-        for k, key in enumerate(VIEW_KEYS):
-            rec[t, k] = float(h[key])
+        rec[t] = _header_values(h, VIEW_KEYS, "view %d" % t)
     return rec
 
 
 def map_record(cube_header):
-    """(12,) float64: CDELT1-3, PSHAPE1-3, MMIN1-3, MMAX1-3 (C_siddon.c.template:1058-1075)."""
-    keys = [p + str(i) for p in ("CDELT", "PSHAPE", "MMIN", "MMAX") for i in (1, 2, 3)]
-    return np.asarray([float(cube_header[k]) for k in keys], dtype=np.float64)
+    """(12,) float64: CDELT1-3, PSHAPE1-3, MMIN1-3, MMAX1-3 (C_siddon.c.template:1058-1075);
+    raises on missing or non-finite values."""
+    return _header_values(cube_header, MAP_KEYS, "map")
 
 
 def pixel_angles(h, n1, n2):

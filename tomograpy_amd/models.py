@@ -25,49 +25,39 @@ from .lo_wrapper import Siddon, Siddon4d
 sigma = 1.0  # models.py:17
 
 
-def _r2omega(r):
-    """Omega = arcsin(1/r), 0 where r == 0 (models.py:237-242)."""
+def billings_ab(r):
+    """Billings' geometric factors A(r) and B(r) of Thomson scattering by an
+    electron r solar radii from the Sun's centre (Billings 1966, "A Guide to
+    the Solar Corona", ch. 6; the C1 and C2 of models.py:264-275).  With
+    s = sin(Omega) = 1/r, Omega the Sun's angular radius seen from the
+    electron, and c = cos(Omega) = sqrt(1 - s^2):
+
+        A = c s^2
+        B = -(1 - 3 s^2 - (1 + 3 s^2) (c^2 / s) ln((1 + s) / c)) / 8
+
+    Both are NaN inside the Sun (r < 1) and at r = 0; the map coefficient
+    zeroes them there, as models.py:307-308 does.
+    """
     r = np.asarray(r, dtype=np.float64)
     with np.errstate(divide="ignore", invalid="ignore"):
-        omega = np.arcsin(1.0 / r)
-    omega = np.where(r == 0, 0.0, omega)
-    return omega
+        s = 1.0 / r
+        s2 = s * s
+        c = np.sqrt(1.0 - s2)
+        a = c * s2
+        log_term = (c * c / s) * np.log((1.0 + s) / c)
+        b = -0.125 * (1.0 - 3.0 * s2 - (1.0 + 3.0 * s2) * log_term)
+    return a, b
 
 
-def _impact_parameter(alpha, beta, d):
-    """rho = d sin(arccos(cos a cos b)), clamped below at 1 (models.py:244-249)."""
-    rho = d * np.sin(np.arccos(np.cos(alpha) * np.cos(beta)))
-    rho = np.asarray(rho, dtype=np.float64)
-    rho[rho < 1.0] = 1.0
-    return rho
-
-
-def _pb_thomson_coef(omega):
-    """Billings' C1 = cos(O) sin(O)^2 and C2 (models.py:264-275)."""
-    with np.errstate(divide="ignore", invalid="ignore"):
-        sino = np.sin(omega)
-        sino2 = sino ** 2
-        coso = np.cos(omega)
-        coso2 = coso ** 2
-        lno = np.log((1.0 + sino) / coso) * (coso2 / sino)
-        C1 = coso * sino2
-        C2 = -(1.0 / 8.0) * (1.0 - 3.0 * sino2 - (1.0 + 3.0 * sino2) * lno)
-    return C1, C2
-
-
-def _thomson_coef(omega):
-    """All four Billings coefficients C1..C4 (models.py:251-262)."""
-    with np.errstate(divide="ignore", invalid="ignore"):
-        sino = np.sin(omega)
-        sino2 = sino ** 2
-        coso = np.cos(omega)
-        coso2 = coso ** 2
-        lno = np.log((1.0 + sino) / coso) * (coso2 / sino)
-        C1 = coso * sino2
-        C2 = -(1.0 / 8.0) * (1.0 - 3.0 * sino2 - (1.0 + 3.0 * sino2) * lno)
-        C3 = 4.0 / 3.0 - coso * (1.0 + coso2 / 3.0)
-        C4 = (1.0 / 8.0) * (5.0 + sino2 - (5.0 - sino2) * lno)
-    return C1, C2, C3, C4
+def impact_parameter_sq(alpha, beta, d):
+    """Squared distance from the Sun's centre to the line of sight of the pixel
+    at angles (alpha, beta) from an observer at distance d, at least 1: the
+    line makes the angle arccos(cos alpha cos beta) with the Sun-observer line,
+    so rho^2 = d^2 (1 - (cos alpha cos beta)^2) (models.py:244-249, whose rho is
+    clamped below at 1 before squaring at models.py:289)."""
+    t = np.cos(alpha) * np.cos(beta)
+    rho2 = np.asarray(d * d * (1.0 - t * t), dtype=np.float64)
+    return np.maximum(rho2, 1.0)
 
 
 def pb_data_coef(data_header, n1, n2):
@@ -76,7 +66,7 @@ def pb_data_coef(data_header, n1, n2):
     for t, h in enumerate(data_header):
         a, b = pixel_angles(h, n1, n2)
         A, B = np.meshgrid(a, b, indexing="ij")
-        out[..., t] = _impact_parameter(A, B, h["D"]) ** 2
+        out[..., t] = impact_parameter_sq(A, B, h["D"])
     return out
 
 
@@ -85,13 +75,11 @@ def pb_map_coef(cube_header, u=0.5, convention="fits"):
     x, y, z = map_axes(cube_header, convention)
     X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
     R2 = X * X + Y * Y + Z * Z
-    R = np.sqrt(R2)
-    O = _r2omega(R)
-    C1, C2 = _pb_thomson_coef(O)
+    a, b = billings_ab(np.sqrt(R2))
     with np.errstate(divide="ignore", invalid="ignore"):
-        coefs = ((1 - u) * C1 + u * C2) / (R2 ** 2)
+        coefs = ((1 - u) * a + u * b) / (R2 * R2)
     coefs[~np.isfinite(coefs)] = 0.0
-    return coefs * np.pi * sigma / 2.0
+    return coefs * (np.pi * sigma / 2.0)
 
 
 def map_radius(cube_header, convention="fits"):
@@ -182,50 +170,114 @@ def smoothness_axes(ndim):
     return tuple(range(ndim))
 
 
-def srt(data, cube, **kwargs):
+class SmoothnessPrior(tuple):
+    """The smoothness priors of a model: the axes k of the differences D_k (a
+    tuple, as ``cg.solve(axes=...)`` takes them) and, for the height prior,
+    the per-voxel weight ``weight`` that each D_k is composed with (None
+    without it)."""
+
+    def __new__(cls, axes, weight=None):
+        self = super().__new__(cls, tuple(axes))
+        self.weight = weight
+        return self
+
+
+def radius_sq_map(cube_header, convention="fits"):
+    """Squared distance of every voxel to the Sun's centre: the _radius_map of
+    models.py:96-104, which returns X^2 + Y^2 + Z^2 without a square root."""
+    x, y, z = map_axes(cube_header, convention)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    return X * X + Y * Y + Z * Z
+
+
+def smoothness_prior(cube_header, ndim=3, height_prior=False, convention="fits"):
+    """Priors D_k = lo.diff(axis=k), k < ndim (models.py:85-94); with
+    ``height_prior`` each composed with diag(r^2), r^2 from radius_sq_map
+    (models.py:90-93).  The reference's _radius_map unpacks three axes, so a
+    height prior on a 4-D map fails there (ValueError); it raises here too."""
+    if not height_prior:
+        return SmoothnessPrior(range(ndim))
+    if ndim != 3:
+        raise ValueError("the height prior is defined for 3-D maps (models.py:96-104)")
+    return SmoothnessPrior(range(3), radius_sq_map(cube_header, convention))
+
+
+def time_bins(data_header, dt_min=None):
+    """Time bin of every image of a stack sorted by DATE_OBS: a new bin starts
+    where an image follows the previous one by dt_min seconds or more, so
+    images closer in time than dt_min share a bin (the grouping
+    solar.temporal_groups describes, solar.py:381-388).  Without dt_min every
+    image has its own bin, as the comment at models.py:125 intends.
+
+    The reference's stsrt counts the short gaps instead (solar.py:368-372 with
+    models.py:126-131): images taken at one time give nv - 1 bins for nv
+    images, and the last image's bin index lies beyond the cube's time axis.
+    """
+    from .solar import convert_time
+    nv = len(data_header)
+    if dt_min is None:
+        return np.arange(nv, dtype=np.int32)
+    times = np.array([convert_time(h["DATE_OBS"]) for h in data_header], dtype=np.float64)
+    if np.any(np.diff(times) < 0):
+        raise ValueError("stsrt expects images sorted by DATE_OBS (solar.sort_data_array)")
+    starts = np.concatenate([[0], (np.diff(times) >= dt_min).astype(np.int32)])
+    return np.cumsum(starts).astype(np.int32)
+
+
+def _object_mask(cube, kwargs):
+    if kwargs.get("obj_rmin") is not None or kwargs.get("obj_rmax") is not None:
+        return define_map_mask(cube.header, **kwargs)
+    return None
+
+
+def srt(data, cube, height_prior=False, **kwargs):
     """Solar rotational tomography model (models.py:19-54).
 
-    Returns (P, prior_axes, obj_mask, data_mask): P the data-masked projector
-    with the Sun obstacle and the voxel mask of obj_rmin/obj_rmax fused in.
+    Returns (P, prior, obj_mask, data_mask): P the data-masked projector with
+    the Sun obstacle and the voxel mask of obj_rmin/obj_rmax fused in; prior a
+    SmoothnessPrior (the axes for ``cg.solve(axes=...)``, with the r^2 weight
+    of the height prior when ``height_prior`` is set).
     """
     data_mask = define_data_mask(data, **kwargs)
-    obj_mask = None
-    if kwargs.get("obj_rmin") is not None or kwargs.get("obj_rmax") is not None:
-        obj_mask = define_map_mask(cube.header, **kwargs)
+    obj_mask = _object_mask(cube, kwargs)
     P = Siddon(data.header, cube.header, mask=data_mask, obstacle="sun", voxmask=obj_mask,
                dtype=kwargs.get("dtype"))
-    return P, smoothness_axes(3), obj_mask, data_mask
+    prior = smoothness_prior(cube.header, 3, height_prior, kwargs.get("convention", "fits"))
+    return P, prior, obj_mask, data_mask
 
 
-def stsrt(data, cube, n_groups=None, tindex=None, **kwargs):
+def stsrt(data, cube, n_groups=None, tindex=None, dt_min=None, height_prior=False, **kwargs):
     """Smooth temporal SRT (models.py:106-153): a 4-D map with one bin per group.
 
-    ``n_groups`` consecutive-image groups (the reference's temporal_groups with
-    equal group sizes), or an explicit per-image ``tindex``.
+    The bins come from an explicit per-image ``tindex``, or ``n_groups``
+    groups of consecutive images of equal size, or the DATE_OBS gaps of the
+    stack with ``dt_min`` (``time_bins``), or else one bin per image.
     """
+    if height_prior:
+        raise ValueError("the height prior is defined for 3-D maps (models.py:96-104)")
     data_mask = define_data_mask(data, **kwargs)
     nv = len(data.header)
     if tindex is None:
-        n_groups = n_groups or nv
-        ng = nv // n_groups
-        tindex = np.minimum(np.arange(nv) // ng, n_groups - 1)
-    obj_mask = None
-    if kwargs.get("obj_rmin") is not None or kwargs.get("obj_rmax") is not None:
-        obj_mask = define_map_mask(cube.header, **kwargs)
+        if n_groups is not None:
+            ng = nv // n_groups
+            tindex = np.minimum(np.arange(nv) // ng, n_groups - 1)
+        else:
+            tindex = time_bins(data.header, dt_min)
+    obj_mask = _object_mask(cube, kwargs)
     P = Siddon4d(data.header, cube.header, tindex=tindex, mask=data_mask, obstacle="sun",
                  voxmask=obj_mask, dtype=kwargs.get("dtype"))
-    return P, smoothness_axes(4), obj_mask, data_mask
+    return P, smoothness_prior(cube.header, 4), obj_mask, data_mask
 
 
 def thomson(data, cube, u=0.5, **kwargs):
-    """Thomson scattering model for white-light coronagraphs (models.py:190-221)."""
+    """Thomson scattering model for white-light coronagraphs (models.py:190-221);
+    its priors are the plain differences (models.py:217), whatever
+    ``height_prior`` says."""
     data_mask = define_data_mask(data, **kwargs)
     pb = kwargs.get("pb", "pb")
     if pb != "pb":
         raise ValueError("Only pb implemented for now.")
-    obj_mask = None
-    if kwargs.get("obj_rmin") is not None or kwargs.get("obj_rmax") is not None:
-        obj_mask = define_map_mask(cube.header, **kwargs)
+    obj_mask = _object_mask(cube, kwargs)
     P = thomson_lo(data.header, cube.header, u=u, mask=data_mask, voxmask=obj_mask,
                    dtype=kwargs.get("dtype"))
-    return P, smoothness_axes(3), obj_mask, data_mask
+    return P, smoothness_prior(cube.header, 3), obj_mask, data_mask

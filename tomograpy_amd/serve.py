@@ -16,8 +16,11 @@ stream-ordered copies, so admissions and completions do not stall the GPU work
 of the other requests; a request is complete when its solution is on the host.
 
 ``serve_plain`` is the per-request baseline: the same requests solved one at
-a time, in arrival order, by ``cg.solve`` (the same kernels, one job at a
-time).  Both report p50/p99 latency (completion - arrival) and throughput.
+a time by ``cg.solve`` (the same kernels, one job at a time), in arrival order
+or in the server's shortest-first order.  Both report p50/p99 latency
+(completion - arrival) and throughput.  A request can be cancelled
+(``Request.cancel``) or given a timeout; either drops it from the queue or the
+active set.
 """
 import collections
 import time
@@ -40,7 +43,8 @@ class Request:
     """
 
     def __init__(self, rid, cube_header, data_header, y, arrival=0.0, lam=1e-2, tol=1e-4,
-                 maxiter=200, mask=None, tindex=None, obstacle=None, dtype=np.float32):
+                 maxiter=200, mask=None, tindex=None, obstacle=None, dtype=np.float32,
+                 timeout=None):
         self.id = rid
         self.cube_header = cube_header
         self.data_header = data_header
@@ -53,19 +57,26 @@ class Request:
         self.tindex = tindex
         self.obstacle = obstacle
         self.dtype = np.dtype(dtype)
+        # seconds after arrival by which the request must be complete; the
+        # server drops it (status "timed out") once that has passed
+        self.timeout = timeout
+        self.status = "queued"  # -> "done", "cancelled" or "timed out"
+        self._cancel = False
         self.x = None
         self.iterations = 0
         self.residual = None
         self.t_admit = None
         self.t_done = None
         self._op = None
+        self._plan_bytes = None
         self._reserved = 0
         self._work = None
         self.x_host = None
 
     def footprint(self):
         """Device bytes while admitted, without the brick plan: 4 CG vectors, 2
-        operator buffers, the float64 backprojection accumulator, images."""
+        operator buffers, the float64 backprojection accumulator, images, the
+        per-ray step counts (4 bytes per pixel) and the pixel mask."""
         n = np.prod([self.cube_header["NAXIS%d" % (k + 1)] for k in range(3)])
         nt = 1 if self.tindex is None else int(np.max(self.tindex)) + 1
         n1 = self.data_header[0]["NAXIS1"]
@@ -74,8 +85,9 @@ class Request:
         b = self.dtype.itemsize
         trig = len(self.data_header) * (n1 + n2) * 16
         acc = 8 * n * nt if self.dtype == np.float32 else 0
-        return int((6 * n * nt + 2 * pix) * b + acc + trig + (pix if self.mask is not None else 0)
-                   + (1 << 16))
+        steps = 4 * pix
+        return int((6 * n * nt + 2 * pix) * b + acc + trig + steps
+                   + (pix if self.mask is not None else 0) + (1 << 16))
 
     def work(self):
         """Relative cost of one CG iteration: rays x mean cube edge."""
@@ -110,6 +122,20 @@ class Request:
     @property
     def latency(self):
         return None if self.t_done is None else self.t_done - self.arrival
+
+    def cancel(self):
+        """Asks the server to drop this request: a waiting request is never
+        admitted, an admitted one leaves at the next engine step and frees its
+        slot and memory; a completed one is unaffected."""
+        self._cancel = True
+
+    def _dropped(self, now):
+        """The status to end with if cancelled or past its timeout at `now`."""
+        if self._cancel:
+            return "cancelled"
+        if self.timeout is not None and now - self.arrival > self.timeout:
+            return "timed out"
+        return None
 
 
 def make_requests(mix, rng, rate=None, tol=1e-4, lam=1e-2, maxiter=200, dtype=np.float32,
@@ -182,12 +208,18 @@ def _download(req, x):
 
 
 def summarize(requests, t_first, t_end):
-    lat = np.array([r.latency for r in requests])
+    """Latency percentiles and throughput over the completed requests."""
+    done = [r for r in requests if r.status == "done"]
+    lat = np.array([r.latency for r in done]) if done else np.zeros(1)
     span = t_end - t_first
-    return {"requests": len(requests), "p50_s": float(np.percentile(lat, 50)),
-            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
-            "makespan_s": span, "throughput_rps": len(requests) / span if span > 0 else float("inf"),
-            "iterations": int(sum(r.iterations for r in requests))}
+    out = {"requests": len(done), "p50_s": float(np.percentile(lat, 50)),
+           "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
+           "makespan_s": span, "throughput_rps": len(done) / span if span > 0 else float("inf"),
+           "iterations": int(sum(r.iterations for r in done))}
+    dropped = len(requests) - len(done)
+    if dropped:
+        out["dropped"] = dropped
+    return out
 
 
 class Server:
@@ -238,12 +270,14 @@ class Server:
         ev.record()
         return host, ev
 
-    def run(self, requests):
-        """Serves `requests` (arrival times in seconds from now); returns a summary."""
+    def run(self, requests, on_step=None):
+        """Serves `requests` (arrival times in seconds from now); returns a summary.
+        ``on_step(step, requests)``, if given, is called after every engine step
+        (between steps a caller may cancel requests)."""
         with S.fixed_stream():
-            return self._run(requests)
+            return self._run(requests, on_step)
 
-    def _run(self, requests):
+    def _run(self, requests, on_step=None):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         active = []  # (req, solve, row)
@@ -260,6 +294,21 @@ class Server:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
+            # cancellations and timeouts: waiting requests leave the queue, admitted
+            # ones leave the active set and give back their slot and memory
+            for r in [r for r in waiting if r._dropped(now)]:
+                waiting.remove(r)
+                r.status, r.t_done = r._dropped(now), now
+            gone = [(r, s, row) for r, s, row in active if r._dropped(now)]
+            for r, s, row in gone:
+                ended.add(id(r))
+                r.status, r.t_done = r._dropped(now), now
+                r.iterations = s.iterations
+                used -= r._reserved
+                self.free_rows.append(row)
+            if gone:
+                active = [a for a in active if id(a[0]) not in ended]
+                table = None
             admitted = []
             if self.order == "shortest" and len(waiting) > 1:
                 waiting = collections.deque(sorted(waiting, key=lambda r: r.work()))
@@ -268,17 +317,23 @@ class Server:
                 fp = req.footprint()
                 if fp > self.budget:
                     raise ValueError("request %s needs %d bytes, budget %d" % (req.id, fp, self.budget))
-                if req._op is None:
+                if req._plan_bytes is None:
+                    # the plan's size needs its record count: build the operator
+                    # (kept if the request is admitted now) and count once
                     req._op = req.operator()
-                P = req._op
-                fp += P.plan_reserve_bytes()
+                    req._plan_bytes = req._op.plan_reserve_bytes()
+                fp += req._plan_bytes
                 req._reserved = fp
                 if fp > self.budget:
                     raise ValueError("request %s needs %d bytes, budget %d" % (req.id, fp, self.budget))
                 if used + fp > self.budget:
+                    # a request that waits holds no device memory: its operator's
+                    # buffers are not counted in `used`
+                    req._op = None
                     break
                 waiting.popleft()
                 used += fp
+                P = req._op if req._op is not None else req.operator()
                 req._op = None
                 row = self.free_rows.pop()
                 s = CG.CGSolve(P, _y_device(req, P, non_blocking=True), lam=req.lam, tol=req.tol,
@@ -302,6 +357,8 @@ class Server:
                     table = CG.SegTable([s for _, s, _ in active], self.dev)
                 CG.iterate([s for _, s, _ in active], table, self.tuner if len(active) > 1 else None)
                 steps += 1
+                if on_step is not None:
+                    on_step(steps, requests)
                 due = [(r, s, row) for r, s, row in active
                        if s.iterations % self.check_every == 0 or s.iterations >= r.maxiter]
                 if due:
@@ -354,25 +411,42 @@ def _collect(finishing, t0, block=False):
         if ev.query():
             r.x = host.numpy()
             r.t_done = time.perf_counter() - t0
+            r.status = "done"
         else:
             still.append((r, host, ev))
     return still
 
 
-def serve_plain(requests, check_every=5):
-    """Per-request baseline: solve each request alone, in arrival order."""
+def serve_plain(requests, check_every=5, order="arrival"):
+    """Per-request baseline: solve each request alone, one after the other.
+    order: "arrival" takes them in arrival order; "shortest" takes, among the
+    requests that have arrived, the one with the least work per iteration
+    first -- the server's admission order, for a like-for-like latency
+    comparison.  A request past its timeout when its turn comes is dropped."""
+    if order not in ("shortest", "arrival"):
+        raise ValueError("order must be 'shortest' or 'arrival'")
     with S.fixed_stream():
-        return _serve_plain(requests, check_every)
+        return _serve_plain(requests, check_every, order)
 
 
-def _serve_plain(requests, check_every):
-    order = sorted(requests, key=lambda r: r.arrival)
+def _serve_plain(requests, check_every, order):
+    pending = sorted(requests, key=lambda r: r.arrival)
     t0 = time.perf_counter()
-    for req in order:
+    while pending:
         now = time.perf_counter() - t0
-        if req.arrival > now:
-            time.sleep(req.arrival - now)
-        req.t_admit = time.perf_counter() - t0
+        if pending[0].arrival > now:
+            time.sleep(pending[0].arrival - now)
+            now = time.perf_counter() - t0
+        req = pending[0]
+        if order == "shortest":
+            arrived = [r for r in pending if r.arrival <= now]
+            req = min(arrived, key=lambda r: r.work())
+        pending.remove(req)
+        gone = req._dropped(now)
+        if gone:
+            req.status, req.t_done = gone, now
+            continue
+        req.t_admit = now
         P = req.operator()
         x, info = CG.solve(P, _y_device(req, P), lam=req.lam, maxiter=req.maxiter, tol=req.tol,
                            check_every=check_every)
@@ -382,5 +456,6 @@ def _serve_plain(requests, check_every):
         req.iterations = info["iterations"]
         req.residual = info["residual"]
         req.t_done = time.perf_counter() - t0
+        req.status = "done"
     t_end = time.perf_counter() - t0
     return summarize(requests, min(r.arrival for r in requests), t_end)

@@ -339,6 +339,23 @@ int tg_batch_plan_table(const tg_call *calls, int32_t njobs, const tg_batch_plan
 int tg_batch_job_layout(int32_t dtype, size_t *job_bytes, size_t *tile_begin_offset,
                         size_t *cube_offset);
 
+/* ---- an entry point that allocates nothing ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tg_conic_full_projector for bpj/bpjt with the float64
+ * accumulation of TG_ACC_F64, whose accumulator is a device array inside the
+ * library (tg_static_acc_capacity() doubles per GPU, held until the process
+ * ends) instead of c->acc, which is ignored.  The cube must be dense (as
+ * TG_ACC_F64 requires) and n0*n1*n2 (times nt for bpjt) at most the capacity,
+ * else TG_EINVAL; TG_DETERMINISTIC is rejected.  pj/pjt calls run as in
+ * tg_conic_full_projector.  The call clears the part of the array it uses
+ * with a kernel, then sums into it; every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other streams, other threads) race
+ * and must be serialised by the caller. */
+int tg_conic_full_projector_static(const tg_call *c, void *stream);
+/* Doubles in the static accumulator (2^24). */
+int64_t tg_static_acc_capacity(void);
+
 /* Thread-local message describing the last error on this thread. */
 const char *tg_last_error(void);
 

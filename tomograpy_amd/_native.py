@@ -109,6 +109,8 @@ SIGNATURES = [
                                            ctypes.POINTER(_sz)]),
     ("tg_batch_run", ctypes.c_int, [ctypes.POINTER(TgCall), _i32, ctypes.POINTER(TgBatchPlan),
                                     _vp, _vp, _vp]),
+    ("tg_conic_full_projector_static", ctypes.c_int, [ctypes.POINTER(TgCall), _vp]),
+    ("tg_static_acc_capacity", _i64, []),
 ]
 
 

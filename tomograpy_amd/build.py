@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libtomograpy_amd.so")
-SOURCES = ["siddon.hip", "siddon_det.hip", "siddon_cg.hip", "siddon_batch.hip", "siddon_plan.hip"]
+SOURCES = ["siddon.hip", "siddon_det.hip", "siddon_cg.hip", "siddon_batch.hip", "siddon_plan.hip",
+           "siddon_static.hip"]
 ARCH = os.environ.get("TOMOGRAPY_AMD_ARCH", "gfx950")
 
 

@@ -52,6 +52,11 @@ int check_acc_jobs(const KArgs<T> *const *args, int njobs) {
   return TG_OK;
 }
 
+// Projects or backprojects views [t0, t1) of *c (ray_i >= 0: the one ray
+// (ray_i, ray_j)); the body of tg_conic_full_projector and its relatives
+// (siddon.hip).
+int dispatch_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, void *stream);
+
 // Deterministic backprojection (siddon_det.hip).
 template <typename T>
 int run_deterministic(const tg_call *c, const KArgs<T> &a, hipStream_t s);

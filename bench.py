@@ -9,10 +9,19 @@ cube buffer, backproject the images (siddon_lo matvec + rmatvec,
 lo_wrapper.py:23-34).  Inputs are resident in HBM before timing starts.
 
 value = Siddon steps per second (voxel segments traversed by the pair / time).
-roofline: the dominant kernel's algorithmic bytes per launch over its average
-launch time (HIP events on the launch stream), against 8 TB/s HBM.
-  forward  bytes = S*b + R_t*b + V*18*b   (voxel gather per step, pixel write per ray)
-  backward bytes = 2*S*b + R_t*b + V*18*b (voxel read-modify-write per step, pixel read per ray)
+roofline: the dominant kernel against the resource that binds it, instruction
+issue: its wave64 VALU instructions per launch (SQ_INSTS_VALU, from the
+committed counter summary profiles/siddon_c2_pair_counters.json when this run
+is the workload it measured) over its average launch time (HIP events on the
+launch stream), against 614.4 G instructions/s (256 CUs x 2.4 GHz x one wave64
+VALU instruction per CU per clock = the 157.3 TFLOP/s packed-FP32 peak / 256
+flops).  Beside it: the HBM bytes per launch from the request-size-resolved
+TCC read/write counters of the same summary against 8 TB/s, the L2 hit rate,
+and the per-step access bytes of the survey's model (access_bytes):
+  forward  access = S*b + R_t*b + V*18*b   (voxel gather per step, pixel write per ray)
+  backward access = 2*S*b + R_t*b + V*18*b (voxel read-modify-write per step, pixel read per ray)
+which the brick backprojection serves from LDS and the forward's gathers from
+L2, so that access rate exceeds the HBM peak and is not a fraction of it.
 cpu_baseline: the oracle (C/OpenMP restatement of the reference) on a bounded
 sample of the same workload's views, on the host cores; "parity" compares the
 GPU pair with the oracle's output on that sample.
@@ -35,6 +44,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# wave64 VALU instructions per second: 256 CUs x 2.4 GHz x 1 per CU per clock
+VALU_PEAK_GIPS = 256 * 2.4
+COUNTERS = os.path.join(ROOT, "profiles", "siddon_c2_pair_counters.json")
 
 
 def parse():
@@ -56,8 +68,11 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="reconstruction stream served vs plain (0 = skip)")
     ap.add_argument("--serve-max-active", type=int, default=8)
-    ap.add_argument("--serve-rate", type=float, default=0.0,
-                    help="Poisson arrival rate in requests/s (0 = all requests at once)")
+    ap.add_argument("--serve-rate", type=float, default=12.0,
+                    help="Poisson arrival rate of the mixed stream in requests/s (0 = all at "
+                         "once); the default is above what one request at a time serves")
+    ap.add_argument("--serve-small-rate", type=float, default=400.0,
+                    help="Poisson arrival rate of the small-request stream (0 = all at once)")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-shard", action="store_true",
@@ -284,9 +299,11 @@ def serving_leg(args, small=False):
     """The mixed-size reconstruction stream (32^3-256^3 cubes, 8-128 views of
     64^2-512^2, pixel mask on/off, 1-8 time bins, CG to a relative residual of
     1e-4), solved once by the continuous-batching server and once one request
-    at a time in arrival order; same requests, same seeds.  small=True: twice
-    as many requests from the small end of the mix (32^3-64^3 cubes, 8-24 views
-    of 64^2-128^2, 1-4 bins) with 32 slots, where a request alone does not fill
+    at a time in the server's admission order (shortest work first among the
+    requests that have arrived); same requests, same seeds, Poisson arrivals at
+    --serve-rate requests/s (0: all at once).  small=True: twice as many
+    requests from the small end of the mix (32^3-64^3 cubes, 8-24 views of
+    64^2-128^2, 1-4 bins) with 32 slots, where a request alone does not fill
     the GPU."""
     import torch
     from tomograpy_amd import serve as SV
@@ -295,6 +312,8 @@ def serving_leg(args, small=False):
     n_req = 2 * args.serve_requests if small else args.serve_requests
     slots = 32 if small else args.serve_max_active
 
+    rate = args.serve_small_rate if small else args.serve_rate
+
     def requests(dtype=np.float32):
         rng = np.random.default_rng(W.SEED0 + (6 if small else 5))
         if small:
@@ -302,48 +321,102 @@ def serving_leg(args, small=False):
                                 max_bins=4)
         else:
             mix = W.config5_mix(n_req, rng)
-        return SV.make_requests(mix, rng, rate=args.serve_rate or None, tol=1e-4, maxiter=200,
-                                dtype=dtype)
+        out, t = [], 0.0
+        for k in range(0, len(mix), 100):
+            part = SV.make_requests(mix[k:k + 100], rng, rate=rate or None, tol=1e-4,
+                                    maxiter=200, dtype=dtype, start=t, first_id=k)
+            out += part
+            t = part[-1].arrival
+            if len(mix) > 100:
+                print("serving: %d of %d requests made" % (len(out), len(mix)), file=sys.stderr,
+                      flush=True)
+        return out
 
-    plain_reqs = requests()
-    served_reqs = requests()
+    reqs = requests()
     torch.cuda.synchronize()
-    print("serving: %d requests made" % len(plain_reqs), file=sys.stderr, flush=True)
+    print("serving: %d requests made" % len(reqs), file=sys.stderr, flush=True)
     tag = "serving_small" if small else "serving"
+    # the plain run's solutions: a digest of each, and whole copies of a sample
+    # (every request of a short stream, 1 in 50 of a long one) for the error
+    import hashlib
+    keep_every = 1 if len(reqs) <= 64 else 50
+    digest, plain_x, plain_its, plain_res = {}, {}, {}, {}
+    last = [time.perf_counter()]
+
+    def progress(what, r):
+        if time.perf_counter() - last[0] > 20.0:
+            last[0] = time.perf_counter()
+            print("serving: %s, request %d done at %.1f s" % (what, r.id, r.t_done),
+                  file=sys.stderr, flush=True)
+
+    def plain_done(r):
+        digest[r.id] = hashlib.sha1(np.ascontiguousarray(r.x).tobytes()).hexdigest()
+        plain_its[r.id], plain_res[r.id] = r.iterations, r.residual
+        if r.id % keep_every == 0:
+            plain_x[r.id] = np.array(r.x)
+        r.x = None
+        progress("plain", r)
+
+    same = {"bits": 0, "iterations": 0}
+    errs = []
+    served_x = {}
+
+    def served_done(r):
+        h = hashlib.sha1(np.ascontiguousarray(r.x).tobytes()).hexdigest()
+        same["bits"] += int(h == digest[r.id])
+        same["iterations"] += int(r.iterations == plain_its[r.id])
+        if r.id in plain_x:
+            b = plain_x[r.id]
+            errs.append(float(np.linalg.norm(r.x - b) / max(np.linalg.norm(b), 1e-300)))
+            served_x[r.id] = np.array(r.x)
+        r.x = None
+        progress("served", r)
+
     # marker ranges (roctx) that a trace of this run can cut its phases by
     torch.cuda.nvtx.range_push(tag + " plain")
-    plain = SV.serve_plain(plain_reqs)
+    plain = SV.serve_plain(reqs, order="shortest", on_complete=plain_done)
     torch.cuda.synchronize()
     torch.cuda.nvtx.range_pop()
     print("serving: plain done in %.1f s" % plain["makespan_s"], file=sys.stderr, flush=True)
+    converged = all(r.residual <= 1e-4 for r in reqs)
+    for r in reqs:
+        r.reset()
     torch.cuda.nvtx.range_push(tag + " served")
-    served = SV.Server(budget_bytes=128 << 30, max_active=slots).run(served_reqs)
+    served = SV.Server(budget_bytes=128 << 30, max_active=slots).run(reqs,
+                                                                       on_complete=served_done)
     torch.cuda.synchronize()
     torch.cuda.nvtx.range_pop()
     print("serving: served done in %.1f s" % served["makespan_s"], file=sys.stderr, flush=True)
-    err = max(float(np.linalg.norm(a.x - b.x) / max(np.linalg.norm(b.x), 1e-300))
-              for a, b in zip(served_reqs, plain_reqs))
-    # the served float32 solutions of the 8 lightest requests against float64
-    # solves of the same requests
-    light = sorted(range(len(served_reqs)), key=lambda k: served_reqs[k].work())[:8]
-    reqs64 = requests(np.float64)
-    SV.serve_plain([reqs64[k] for k in light])
-    err64 = [float(np.linalg.norm(served_reqs[k].x - reqs64[k].x) / np.linalg.norm(reqs64[k].x))
-             for k in light]
+    converged = converged and all(r.residual <= 1e-4 for r in reqs)
+    # the served float32 solutions of the 8 lightest sampled requests against
+    # float64 solves of the same requests (same images, upcast)
+    light = sorted(served_x, key=lambda k: reqs[k].work())[:8]
+    reqs64 = [SV.Request(k, reqs[k].cube_header, reqs[k].data_header,
+                         torch.as_tensor(reqs[k].y).double(), lam=reqs[k].lam, tol=reqs[k].tol,
+                         maxiter=reqs[k].maxiter, mask=reqs[k].mask, tindex=reqs[k].tindex,
+                         obstacle=reqs[k].obstacle, dtype=np.float64).pin() for k in light]
+    SV.serve_plain(reqs64)
+    err64 = [float(np.linalg.norm(served_x[k] - r.x) / np.linalg.norm(r.x))
+             for k, r in zip(light, reqs64)]
     del reqs64
-    out = {"requests": len(plain_reqs), "arrival_rate_rps": args.serve_rate or None,
-            "max_active": slots, "tol": 1e-4,
-            "served": served, "plain": plain,
-            "throughput_ratio": served["throughput_rps"] / plain["throughput_rps"],
-            "p50_ratio": plain["p50_s"] / served["p50_s"],
-            "max_rel_l2_served_vs_plain": err,
-            "error_vs_fp64": {"requests": len(light), "solution_rel_l2_max": max(err64),
-                              "solution_rel_l2_median": float(np.median(err64))},
-            "same_iterations": all(a.iterations == b.iterations
-                                   for a, b in zip(served_reqs, plain_reqs)),
-            "all_converged": all(r.residual <= 1e-4 for r in served_reqs + plain_reqs)}
-    if small:
-        out["workload"] = "32^3-64^3 cubes, 8-24 views of 64^2-128^2, mask on/off, 1-4 bins"
+    out = {"workload": ("%d requests, %s, CG to a relative residual of 1e-4, arrivals %s"
+                        % (len(reqs),
+                           "32^3-64^3 cubes, 8-24 views of 64^2-128^2, mask on/off, 1-4 bins"
+                           if small else
+                           "32^3-256^3 cubes, 8-128 views of 64^2-512^2, mask on/off, 1-8 bins",
+                           "Poisson at %g/s" % rate if rate else "all at t = 0")),
+           "requests": len(reqs), "arrival_rate_rps": rate or None,
+           "max_active": slots, "tol": 1e-4, "plain_order": "shortest work first",
+           "served": served, "plain": plain,
+           "throughput_ratio": served["throughput_rps"] / plain["throughput_rps"],
+           "p50_ratio": plain["p50_s"] / served["p50_s"],
+           "p99_ratio": plain["p99_s"] / served["p99_s"],
+           "bit_identical_to_plain": same["bits"], "same_iterations": same["iterations"],
+           "max_rel_l2_served_vs_plain_sampled": max(errs) if errs else None,
+           "sampled": len(errs),
+           "error_vs_fp64": {"requests": len(light), "solution_rel_l2_max": max(err64),
+                             "solution_rel_l2_median": float(np.median(err64))},
+           "all_converged": converged}
     return out
 
 
@@ -398,21 +471,49 @@ def det_leg(args, reps=2):
     return out
 
 
-def measured_traffic(kernel, args, plan_info):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, when this run is the workload that summary measured
-    (counters cannot be read from inside the timed process)."""
-    path = os.path.join(ROOT, "profiles", "siddon_c2_pair_hbm_traffic.json")
-    if not os.path.exists(path):
+def measured_counters(kernel, args, plan_info):
+    """Per-launch counter figures of `kernel` (VALU instructions, HBM bytes, L2
+    hits and misses, atomics) from the committed rocprofv3 summary, when this
+    run is the workload that summary measured (counters cannot be read from
+    inside the timed process); None otherwise."""
+    if not os.path.exists(COUNTERS):
         return None
-    rec = json.load(open(path))
+    rec = json.load(open(COUNTERS))
     w = rec["workload"]
     if (w["n"], w["views"], w["pix"], w["dtype"]) != (args.n, args.views, args.pix, args.dtype):
         return None
     if plan_info is not None and list(w.get("brick", [])) != plan_info["brick"]:
         return None
-    k = rec["kernels"].get(kernel)
-    return None if k is None else k["hbm_bytes_per_launch"]
+    # one backprojection call launches the brick kernel and the float64
+    # accumulator finish, which the bench's events time together
+    return rec["kernels"].get(kernel + "_call", rec["kernels"].get(kernel))
+
+
+def roofline(kernel, t_launch, access_bytes, args, plan_info):
+    """The roofline object of the bench line for `kernel` at t_launch seconds
+    per launch (see the module docstring)."""
+    c = measured_counters(kernel, args, plan_info)
+    out = {"bound": "valu", "kernel": kernel, "unit": "G VALU instructions/s",
+           "peak": VALU_PEAK_GIPS, "achieved": None, "frac": None, "traffic": None,
+           "access_bytes": access_bytes, "access_GBs": access_bytes / t_launch / 1e9}
+    if c is None:
+        out["note"] = "no counter summary for this workload (profiles/ covers the default)"
+        return out
+    valu = c["valu_instructions"]
+    hbm = c["hbm_read_bytes"] + c["hbm_write_bytes"]
+    out.update({"achieved": valu / t_launch / 1e9,
+                "frac": valu / t_launch / 1e9 / VALU_PEAK_GIPS,
+                "traffic": hbm,
+                "valu_instructions_per_launch": valu,
+                "hbm": {"achieved_GBs": hbm / t_launch / 1e9, "peak_GBs": HBM_PEAK_GBS,
+                        "frac": hbm / t_launch / 1e9 / HBM_PEAK_GBS,
+                        "read_bytes": c["hbm_read_bytes"], "write_bytes": c["hbm_write_bytes"]},
+                "l2_hit_rate": c["l2_hit_rate"],
+                "counters": os.path.relpath(COUNTERS, ROOT)})
+    for k in ("lds_instructions", "l2_atomics", "issue_frac_at_measured_clock", "clock_GHz"):
+        if k in c:
+            out[k] = c[k]
+    return out
 
 
 def cpu_baseline(ch, hs, cube_np, data_np, target_s):
@@ -566,7 +667,26 @@ def main():
         dom, B_dom, t_dom = "siddon_backproject", B_bwd, t_bwd
     else:
         dom, B_dom, t_dom = "siddon_project", B_fwd, t_fwd
-    achieved = B_dom / t_dom / 1e9
+    roof = roofline(dom, t_dom, B_dom, args, plan_info)
+    roof_other = roofline("siddon_project" if dom == "siddon_backproject" else "siddon_backproject",
+                          t_fwd if dom == "siddon_backproject" else t_bwd,
+                          B_fwd if dom == "siddon_backproject" else B_bwd, args, plan_info)
+
+    # the pair from a cold operator: per-ray step counts, plan build, one pair
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Pc = siddon_lo(hs, ch, dtype=dtype, tile_j=args.tile_j)
+    Pc.ray_steps()
+    if Pc.use_plan():
+        Pc.build_plan()
+    Pc.xout.zero_()
+    Pc.project_into(x, Pc.xout)
+    Pc.xin.zero_()
+    Pc.backproject_into(y, Pc.xin)
+    torch.cuda.synchronize()
+    ms_cold = (time.perf_counter() - t0) * 1e3
+    del Pc
+    torch.cuda.empty_cache()
 
     extra = {}
     # the float32 results whose error against float64 the fp64 leg states
@@ -583,6 +703,15 @@ def main():
     if args.cg_iters:
         keep = {} if ref32 is not None else None
         extra["cg"] = {"fp32": cg_run(P, y, args.cg_iters, keep)}
+        # the same solve from a cold operator: step counts and plan build included
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Pc = siddon_lo(hs, ch, dtype=dtype, tile_j=args.tile_j)
+        cg_run(Pc, y, args.cg_iters)
+        torch.cuda.synchronize()
+        extra["cg"]["fp32"]["ms_total_with_setup"] = (time.perf_counter() - t0) * 1e3
+        del Pc
+        torch.cuda.empty_cache()
         if keep:
             ref32["cg_x"] = keep["x"]
             # the same solve with the axis views' pixels masked out
@@ -640,12 +769,11 @@ def main():
                                % (args.n, V, args.pix),
                    "siddon_steps_per_projection": S, "rays_traced": R_t,
                    "ms_forward": t_fwd * 1e3, "ms_backward": t_bwd * 1e3,
+                   "ms_cold_pair": ms_cold,
                    "backprojection": "brick plan" if plan_info else "atomic scatter",
                    "plan": plan_info},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom, args, plan_info),
-                     "algorithmic_bytes_per_launch": B_dom},
+        "roofline": roof,
+        "roofline_other_kernel": roof_other,
         "cpu_baseline": cpu,
         "wall_s_timed": t_wall,
     }

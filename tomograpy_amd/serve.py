@@ -63,6 +63,7 @@ class Request:
         self.status = "queued"  # -> "done", "cancelled" or "timed out"
         self._cancel = False
         self.x = None
+        self._pinned = False
         self.iterations = 0
         self.residual = None
         self.t_admit = None
@@ -103,13 +104,30 @@ class Request:
         return n if self.tindex is None else n + (int(np.max(self.tindex)) + 1,)
 
     def pin(self):
-        """Keeps the image stack, and a buffer for the solution, in pinned host
-        memory (asynchronous copies)."""
+        """Keeps the image stack in pinned host memory (asynchronous upload); the
+        solution goes to a pinned buffer taken when the request completes (from
+        torch's caching host allocator, so that buffers freed by earlier
+        requests are reused)."""
         if torch.cuda.is_available():
             y = torch.as_tensor(np.asarray(self.y))
             self.y = y if y.is_pinned() else y.pin_memory()
-            self.x_host = torch.empty(self.result_shape(), dtype=_torch_dtype(self.dtype),
-                                      pin_memory=True)
+            self._pinned = True
+        return self
+
+    def reset(self):
+        """Forgets the outcome of a run (solution, counts, times, status), so that
+        the same request can be served again."""
+        self.x = None
+        self.x_host = None
+        self.iterations = 0
+        self.residual = None
+        self.t_admit = None
+        self.t_done = None
+        self.status = "queued"
+        self._cancel = False
+        self._op = None
+        self._plan_bytes = None
+        self._reserved = 0
         return self
 
     def operator(self):
@@ -139,18 +157,21 @@ class Request:
 
 
 def make_requests(mix, rng, rate=None, tol=1e-4, lam=1e-2, maxiter=200, dtype=np.float32,
-                  noise=0.01):
+                  noise=0.01, start=0.0, first_id=0):
     """Requests of the served stream from a size mix (workloads.config5_mix).
 
     Each request images a seeded blob corona (time bins scale it by 1 + 0.05 t)
     from random observers at the mix's distance; y = P x + noise * max * N(0, 1).
-    Arrival times are a Poisson process of the given rate (all at 0 if None).
+    Arrival times are a Poisson process of the given rate (all at 0 if None)
+    starting at `start` seconds; ids count from first_id.  Calling it on
+    consecutive slices of a mix with the same rng, each call starting at the
+    previous call's last arrival, gives the requests of one call on the whole.
     """
     from . import models as Mo
     from . import workloads as W
     reqs = []
-    t = 0.0
-    for k, spec in enumerate(mix):
+    t = float(start)
+    for k, spec in enumerate(mix, start=first_id):
         ch = W.cube_header(spec["n"], dtype=dtype)
         lons = rng.uniform(0, 2 * np.pi, spec["nv"])
         lats = np.deg2rad(rng.uniform(-7.0, 7.0, spec["nv"]))
@@ -200,7 +221,7 @@ def _download(req, x):
     returns (host tensor, completion event)."""
     host = req.x_host
     if host is None or tuple(host.shape) != tuple(x.shape):
-        host = torch.empty(tuple(x.shape), dtype=x.dtype)
+        host = torch.empty(tuple(x.shape), dtype=x.dtype, pin_memory=req._pinned)
     host.copy_(x, non_blocking=host.is_pinned())
     ev = torch.cuda.Event()
     ev.record()
@@ -270,14 +291,15 @@ class Server:
         ev.record()
         return host, ev
 
-    def run(self, requests, on_step=None):
+    def run(self, requests, on_step=None, on_complete=None):
         """Serves `requests` (arrival times in seconds from now); returns a summary.
         ``on_step(step, requests)``, if given, is called after every engine step
-        (between steps a caller may cancel requests)."""
+        (between steps a caller may cancel requests); ``on_complete(request)``
+        when a request's solution has reached the host."""
         with S.fixed_stream():
-            return self._run(requests, on_step)
+            return self._run(requests, on_step, on_complete)
 
-    def _run(self, requests, on_step=None):
+    def _run(self, requests, on_step=None, on_complete=None):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         active = []  # (req, solve, row)
@@ -290,7 +312,8 @@ class Server:
         ended = set()
         while pending or waiting or active or finishing or checks:
             finishing = _collect(finishing, t0,
-                                 block=not (pending or waiting or active or checks))
+                                 block=not (pending or waiting or active or checks),
+                                 on_complete=on_complete)
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
@@ -401,7 +424,7 @@ class Server:
         return out
 
 
-def _collect(finishing, t0, block=False):
+def _collect(finishing, t0, block=False, on_complete=None):
     """Completes the requests whose solutions have reached the host; with
     block=True waits for the oldest.  Returns the ones still in flight."""
     if finishing and block:
@@ -412,12 +435,14 @@ def _collect(finishing, t0, block=False):
             r.x = host.numpy()
             r.t_done = time.perf_counter() - t0
             r.status = "done"
+            if on_complete is not None:
+                on_complete(r)
         else:
             still.append((r, host, ev))
     return still
 
 
-def serve_plain(requests, check_every=5, order="arrival"):
+def serve_plain(requests, check_every=5, order="arrival", on_complete=None):
     """Per-request baseline: solve each request alone, one after the other.
     order: "arrival" takes them in arrival order; "shortest" takes, among the
     requests that have arrived, the one with the least work per iteration
@@ -426,10 +451,10 @@ def serve_plain(requests, check_every=5, order="arrival"):
     if order not in ("shortest", "arrival"):
         raise ValueError("order must be 'shortest' or 'arrival'")
     with S.fixed_stream():
-        return _serve_plain(requests, check_every, order)
+        return _serve_plain(requests, check_every, order, on_complete)
 
 
-def _serve_plain(requests, check_every, order):
+def _serve_plain(requests, check_every, order, on_complete=None):
     pending = sorted(requests, key=lambda r: r.arrival)
     t0 = time.perf_counter()
     while pending:
@@ -457,5 +482,7 @@ def _serve_plain(requests, check_every, order):
         req.residual = info["residual"]
         req.t_done = time.perf_counter() - t0
         req.status = "done"
+        if on_complete is not None:
+            on_complete(req)
     t_end = time.perf_counter() - t0
     return summarize(requests, min(r.arrival for r in requests), t_end)

@@ -67,11 +67,15 @@ def parse():
                     help="run one plain device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="reconstruction stream served vs plain (0 = skip)")
-    ap.add_argument("--serve-max-active", type=int, default=8)
+    ap.add_argument("--serve-max-active", type=int, default=32)
+    ap.add_argument("--serve-step-work", type=float, default=0.0,
+                    help="work per engine step of the server (0 = its default)")
     ap.add_argument("--serve-rate", type=float, default=12.0,
                     help="Poisson arrival rate of the mixed stream in requests/s (0 = all at "
                          "once); the default is above what one request at a time serves")
-    ap.add_argument("--serve-small-rate", type=float, default=400.0,
+    ap.add_argument("--serve-small-requests", type=int, default=64,
+                    help="requests of the small-request stream (0 = skip)")
+    ap.add_argument("--serve-small-rate", type=float, default=1000.0,
                     help="Poisson arrival rate of the small-request stream (0 = all at once)")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
@@ -301,15 +305,15 @@ def serving_leg(args, small=False):
     1e-4), solved once by the continuous-batching server and once one request
     at a time in the server's admission order (shortest work first among the
     requests that have arrived); same requests, same seeds, Poisson arrivals at
-    --serve-rate requests/s (0: all at once).  small=True: twice as many
-    requests from the small end of the mix (32^3-64^3 cubes, 8-24 views of
-    64^2-128^2, 1-4 bins) with 32 slots, where a request alone does not fill
-    the GPU."""
+    --serve-rate requests/s (0: all at once).  small=True: the small end of
+    the mix (32^3-64^3 cubes, 8-24 views of 64^2-128^2, 1-4 bins) with 32
+    slots, where a request alone does not fill the GPU, --serve-small-requests
+    of them at --serve-small-rate."""
     import torch
     from tomograpy_amd import serve as SV
     from tomograpy_amd import workloads as W
 
-    n_req = 2 * args.serve_requests if small else args.serve_requests
+    n_req = args.serve_small_requests if small else args.serve_requests
     slots = 32 if small else args.serve_max_active
 
     rate = args.serve_small_rate if small else args.serve_rate
@@ -336,12 +340,15 @@ def serving_leg(args, small=False):
     torch.cuda.synchronize()
     print("serving: %d requests made" % len(reqs), file=sys.stderr, flush=True)
     tag = "serving_small" if small else "serving"
-    # the plain run's solutions: a digest of each, and whole copies of a sample
-    # (every request of a short stream, 1 in 50 of a long one) for the error
-    import hashlib
+    # the plain run's solutions: a device fingerprint of each (two sums over its
+    # bit patterns, taken before the download in both paths), and the host
+    # solutions of a sample (every request of a short stream, 1 in 50 of a
+    # long one) for the error; nothing is copied or hashed inside the runs
     keep_every = 1 if len(reqs) <= 64 else 50
     digest, plain_x, plain_its, plain_res = {}, {}, {}, {}
     last = [time.perf_counter()]
+    for r in reqs:
+        r.fingerprint = True
 
     def progress(what, r):
         if time.perf_counter() - last[0] > 20.0:
@@ -350,25 +357,19 @@ def serving_leg(args, small=False):
                   file=sys.stderr, flush=True)
 
     def plain_done(r):
-        digest[r.id] = hashlib.sha1(np.ascontiguousarray(r.x).tobytes()).hexdigest()
+        digest[r.id] = r.fingerprint
         plain_its[r.id], plain_res[r.id] = r.iterations, r.residual
         if r.id % keep_every == 0:
-            plain_x[r.id] = np.array(r.x)
+            plain_x[r.id] = r.x  # the pinned buffer itself, kept
         r.x = None
         progress("plain", r)
 
-    same = {"bits": 0, "iterations": 0}
-    errs = []
-    served_x = {}
+    served_fp, served_x = {}, {}
 
     def served_done(r):
-        h = hashlib.sha1(np.ascontiguousarray(r.x).tobytes()).hexdigest()
-        same["bits"] += int(h == digest[r.id])
-        same["iterations"] += int(r.iterations == plain_its[r.id])
+        served_fp[r.id] = r.fingerprint
         if r.id in plain_x:
-            b = plain_x[r.id]
-            errs.append(float(np.linalg.norm(r.x - b) / max(np.linalg.norm(b), 1e-300)))
-            served_x[r.id] = np.array(r.x)
+            served_x[r.id] = r.x
         r.x = None
         progress("served", r)
 
@@ -381,22 +382,37 @@ def serving_leg(args, small=False):
     converged = all(r.residual <= 1e-4 for r in reqs)
     for r in reqs:
         r.reset()
+        r.fingerprint = True
     torch.cuda.nvtx.range_push(tag + " served")
-    served = SV.Server(budget_bytes=128 << 30, max_active=slots).run(reqs,
-                                                                       on_complete=served_done)
+    server = SV.Server(budget_bytes=128 << 30, max_active=slots,
+                       step_work=args.serve_step_work or None)
+    served = server.run(reqs, on_complete=served_done)
     torch.cuda.synchronize()
     torch.cuda.nvtx.range_pop()
     print("serving: served done in %.1f s" % served["makespan_s"], file=sys.stderr, flush=True)
     converged = converged and all(r.residual <= 1e-4 for r in reqs)
+    same = {"bits": sum(int(torch.equal(served_fp[k], digest[k])) for k in digest),
+            "iterations": sum(int(r.iterations == plain_its[r.id]) for r in reqs)}
+    errs = [float(np.linalg.norm(served_x[k] - plain_x[k]) / max(np.linalg.norm(plain_x[k]), 1e-300))
+            for k in plain_x]
     # the served float32 solutions of the 8 lightest sampled requests against
     # float64 solves of the same requests (same images, upcast)
     light = sorted(served_x, key=lambda k: reqs[k].work())[:8]
+    light_x = {k: np.array(served_x[k]) for k in light}
     reqs64 = [SV.Request(k, reqs[k].cube_header, reqs[k].data_header,
                          torch.as_tensor(reqs[k].y).double(), lam=reqs[k].lam, tol=reqs[k].tol,
                          maxiter=reqs[k].maxiter, mask=reqs[k].mask, tindex=reqs[k].tindex,
-                         obstacle=reqs[k].obstacle, dtype=np.float64).pin() for k in light]
+                         obstacle=reqs[k].obstacle, dtype=np.float64) for k in light]
+    # release the stream's pinned images and the pinned solution buffers that
+    # torch's host allocator keeps cached (tens of GB for a long stream)
+    keep_y = {k: None for k in light}
+    for r in reqs:
+        r.y = keep_y.get(r.id, None)
+    plain_x = served_x = None
+    if hasattr(torch._C, "_host_emptyCache"):
+        torch._C._host_emptyCache()
     SV.serve_plain(reqs64)
-    err64 = [float(np.linalg.norm(served_x[k] - r.x) / np.linalg.norm(r.x))
+    err64 = [float(np.linalg.norm(light_x[k] - r.x) / np.linalg.norm(r.x))
              for k, r in zip(light, reqs64)]
     del reqs64
     out = {"workload": ("%d requests, %s, CG to a relative residual of 1e-4, arrivals %s"
@@ -406,7 +422,8 @@ def serving_leg(args, small=False):
                            "32^3-256^3 cubes, 8-128 views of 64^2-512^2, mask on/off, 1-8 bins",
                            "Poisson at %g/s" % rate if rate else "all at t = 0")),
            "requests": len(reqs), "arrival_rate_rps": rate or None,
-           "max_active": slots, "tol": 1e-4, "plain_order": "shortest work first",
+           "max_active": slots, "step_work": server.step_work, "tol": 1e-4,
+           "plain_order": "shortest work first",
            "served": served, "plain": plain,
            "throughput_ratio": served["throughput_rps"] / plain["throughput_rps"],
            "p50_ratio": plain["p50_s"] / served["p50_s"],
@@ -416,7 +433,8 @@ def serving_leg(args, small=False):
            "sampled": len(errs),
            "error_vs_fp64": {"requests": len(light), "solution_rel_l2_max": max(err64),
                              "solution_rel_l2_median": float(np.median(err64))},
-           "all_converged": converged}
+           "all_converged": converged,
+           "not_converged_at_maxiter": sum(1 for r in reqs if r.residual > 1e-4)}
     return out
 
 
@@ -721,7 +739,8 @@ def main():
             del Pm
 
     y_host = y.cpu().numpy() if not args.no_cpu else None
-    if args.serve_requests or not args.no_grad or not args.no_batch or args.cg_iters \
+    if args.serve_requests or args.serve_small_requests or not args.no_grad \
+            or not args.no_batch or args.cg_iters \
             or not args.no_shard or not args.no_det:
         # release the pair's operator (its brick plan is 9.4 GB at 256^3) first
         del P, x, x_true, y, noise
@@ -744,6 +763,7 @@ def main():
         extra["deterministic"] = det_leg(args)
     if args.serve_requests:
         extra["serving"] = serving_leg(args)
+    if args.serve_small_requests:
         extra["serving_small"] = serving_leg(args, small=True)
     cpu = None
     if not args.no_cpu:

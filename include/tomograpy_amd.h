@@ -194,13 +194,22 @@ typedef struct tg_bpj_plan {
                             0 = trace order).  Spreading them puts rays that are not
                             neighbours in one wavefront, which pays when a voxel spans
                             several pixels: adjacent rays then add into the same LDS cells */
-  int32_t pad;
+  int32_t pairs_ready;   /* set by tg_bpj_plan_count: 1 when it left each trace workgroup's
+                            per-brick record counts in scratch for tg_bpj_plan_build */
   int64_t records;       /* set by tg_bpj_plan_count */
   int64_t chunks;        /* set by tg_bpj_plan_count */
   void *meta;            /* device, caller-owned, meta_bytes */
   size_t meta_bytes;     /* required size, set by tg_bpj_plan_init */
   void *rec;             /* device, caller-owned, rec_bytes */
   size_t rec_bytes;      /* required size, set by tg_bpj_plan_count */
+  void *scratch;         /* device, caller-owned, or NULL: scratch_bytes for tg_bpj_plan_count
+                            to hand each trace workgroup's per-brick record counts to
+                            tg_bpj_plan_build, which then traces every ray once instead of
+                            twice; needed only between the two calls */
+  size_t scratch_bytes;  /* size that allows it, set by tg_bpj_plan_init (0: not offered) */
+  uint32_t *ray_steps_out; /* device or NULL: tg_bpj_plan_count also writes every ray's Siddon
+                              step count there, as tg_count_steps does (for the pj/pjt call
+                              with the same views, mask and obstacle) */
 } tg_bpj_plan;
 
 /* Fills the derived fields and meta_bytes for the geometry of *c (kind bpj or

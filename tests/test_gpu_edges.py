@@ -94,3 +94,53 @@ def test_cube_edge_1024(oracle, dtype):
     check_pair(oracle, ch, hs, shape, dtype)
     P = siddon_lo(hs, ch, dtype=dtype)
     assert int(P.ray_steps().max()) >= 1024  # the rays inside the slab cross all of it
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", ["c1_sun", "masked", "time_bins", "axis_views", "missing"])
+def test_plan_count_step_counts_and_single_trace_fill(oracle, dtype, case):
+    """The plan's count pass writes every ray's step count as tg_count_steps
+    does (the projector runs on them), and the fill that reuses the count
+    pass's per-workgroup record counts builds the plan the two-trace fill
+    builds: same record count, backprojection equal to the ulp (float64 sums)
+    and within tolerance of the oracle."""
+    from tomograpy_amd import workloads as W
+    from tomograpy_amd.lo_wrapper import Siddon4d
+    print("plan count", case, np.dtype(dtype).name, flush=True)
+    kw, cls = {}, siddon_lo
+    if case == "c1_sun":
+        ch, _, hs = W.config1(dtype)
+        kw["obstacle"] = "sun"
+    elif case == "axis_views":
+        ch, hs, _ = W.config2_geometry(dtype, n=64, n_views=8, n_pix=96)  # LON = k pi / 4
+    else:
+        ch = T.centered_cubic_map_header(3.0, 40, dtype=dtype)
+        ih = T.centered_image_header(T.fov(ch, 30.0), 48, dtype=dtype)
+        if case == "missing":
+            ih["CRVAL1"] = 0.3
+        hs = T.circular_trajectory_header(radius=30.0, longitudes=0.3 + 0.9 * np.arange(6), **ih)
+        if case == "masked":
+            rng = np.random.default_rng(3)
+            kw["mask"] = rng.uniform(size=(48, 48, 6)) < 0.3
+        if case == "time_bins":
+            cls = lambda h, c, **k: Siddon4d(h, c, tindex=np.arange(6) % 3, **k)
+    P1 = cls(hs, ch, dtype=dtype, bpj_mode="plan", **kw)
+    P2 = cls(hs, ch, dtype=dtype, bpj_mode="plan", plan_single_trace=False, **kw)
+    P0 = cls(hs, ch, dtype=dtype, bpj_mode="atomic", **kw)
+    steps_plan = P1.ray_steps().clone()  # written by the plan's count pass
+    assert P1._plan_count is not None
+    assert torch.equal(steps_plan, P0.ray_steps())
+    p1, p2 = P1.build_plan(), P2.build_plan()
+    assert p1.records == p2.records and p1.chunks == p2.chunks
+    rng = np.random.default_rng(4)
+    y = torch.from_numpy(rng.uniform(size=P1.shapeout).astype(dtype)).cuda()
+    b1 = P1.rmatvec(y).cpu().numpy()
+    b2 = P2.rmatvec(y).cpu().numpy()
+    if P1.acc64:  # float64 sums rounded once: equal but for one-ulp ties
+        d = b1 != b2
+        assert d.mean() <= 1e-3
+        assert np.all(np.abs(b1 - b2)[d] <= np.spacing(np.maximum(np.abs(b1), np.abs(b2)))[d])
+    else:  # float64 data: the brick sums' order follows the record order, which may differ
+        assert np.linalg.norm(b1 - b2) <= 1e-13 * max(np.linalg.norm(b2), 1e-300)
+    if case == "missing":
+        assert p1.records == 0 and not b1.any()

@@ -61,8 +61,9 @@ class TgCgSeg(ctypes.Structure):
 class TgBpjPlan(ctypes.Structure):
     _fields_ = [("brick", _i32 * 3), ("chunk_records", _i32), ("nb_axis", _i32 * 3),
                 ("nbricks", _i32), ("order", _i32), ("trace_tile", _i32),
-                ("tie_spread", _i32), ("pad", _i32), ("records", _i64), ("chunks", _i64), ("meta", _vp),
-                ("meta_bytes", _sz), ("rec", _vp), ("rec_bytes", _sz)]
+                ("tie_spread", _i32), ("pairs_ready", _i32), ("records", _i64), ("chunks", _i64),
+                ("meta", _vp), ("meta_bytes", _sz), ("rec", _vp), ("rec_bytes", _sz),
+                ("scratch", _vp), ("scratch_bytes", _sz), ("ray_steps_out", _vp)]
 
 
 class TgBatchPlan(ctypes.Structure):

@@ -124,12 +124,16 @@ __device__ __forceinline__ int brick_of(const BrickGeom &g, const int iv[3], int
 // Traces the ray of (tile, lane); calls emit(brick, ray, id, first) at the
 // first step in every brick and close(steps) when the ray leaves that brick
 // (or the map, or meets the Sun) with the number of steps taken in it.  Same
-// traversal as the projector kernels.
+// traversal as the projector kernels.  With per_ray (not null) it also writes
+// the ray's total step count there, indexed (t * n1 + i) * n2 + j as
+// tg_count_steps writes it (0 for a masked ray or one that misses the map).
 template <typename T, bool SUN, class F, class G>
 __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom &g, uint32_t tile,
-                                             F &&emit, G &&close) {
+                                             F &&emit, G &&close, uint32_t *per_ray = nullptr) {
   int i, j, t;
   if (!lane_pixel(a, tile, i, j, t)) return;
+  uint32_t *out = per_ray ? per_ray + ((size_t)t * a.n1 + i) * a.n2 + j : nullptr;
+  if (out) *out = 0u;
   if (a.pixmask && a.pixmask[i * a.ms[0] + j * a.ms[1] + (int64_t)t * a.ms[2]] != 0) return;
   const ViewHdr<T> &h = a.views[t];
   T u0[3];
@@ -150,7 +154,7 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
   }
   const uint32_t rid = ray_id((uint32_t)a.n1, (uint32_t)a.n2, t, i, j);
   int last = -1;
-  uint32_t seg = 0;
+  uint32_t seg = 0, total = 0;
   int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
   while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
     if (--guard < 0) {
@@ -167,52 +171,94 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
     bool sx, sy, sz;
     step(r, sx, sy, sz);
     seg++;
+    total++;
   }
   if (last >= 0) close(seg);
+  if (out) *out = total;
 }
 
+// Per-workgroup record counts the count pass hands to the fill pass, so that
+// the fill traces every ray once: for workgroup w of the plan's trace grid,
+// a header word (the number of bricks its rays enter, or kPairsOverflow) and
+// up to kWgPairs (brick, records) pairs.
+constexpr int kWgPairs = 255;
+constexpr uint32_t kPairsOverflow = 0xffffffffu;
+constexpr size_t kWgPairBytes = 8 * (size_t)(kWgPairs + 1);
+
+// pairs: the workgroup's kWgPairBytes slot of per-workgroup pairs (or null);
+// per_ray: per-ray step counts out (or null).
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom g,
-                                                           uint32_t *counts) {
+                                                           uint32_t *counts, uint32_t *pairs,
+                                                           uint32_t *per_ray) {
   __shared__ uint32_t hist[kHistMax];
+  __shared__ uint32_t npairs;
   const bool lds = g.nbricks <= kHistMax;
   if (lds) {
     for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) hist[k] = 0;
+    if (threadIdx.x == 0) npairs = 0;
     __syncthreads();
   }
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  if (pairs) pairs += (size_t)blockIdx.x * (kWgPairBytes / 4);
   trace_bricks<T, SUN>(
       a, g, tile,
       [&](int b, const Ray<T> &, uint32_t, bool) {
         if (lds) atomicAdd(&hist[b], 1u);
         else atomicAdd(&counts[b], 1u);
       },
-      [](uint32_t) {});
+      [](uint32_t) {}, per_ray);
   if (lds) {
     __syncthreads();
-    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x)
-      if (hist[k]) atomicAdd(&counts[k], hist[k]);
+    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) {
+      const uint32_t h = hist[k];
+      if (!h) continue;
+      atomicAdd(&counts[k], h);
+      if (pairs) {
+        const uint32_t q = atomicAdd(&npairs, 1u);
+        if (q < (uint32_t)kWgPairs) {
+          pairs[2 + 2 * q] = (uint32_t)k;
+          pairs[3 + 2 * q] = h;
+        }
+      }
+    }
+    __syncthreads();
+    if (pairs && threadIdx.x == 0) pairs[0] = npairs <= (uint32_t)kWgPairs ? npairs : kPairsOverflow;
+  } else if (pairs && threadIdx.x == 0) {
+    pairs[0] = kPairsOverflow;
   }
 }
 
+// pairs: this workgroup's slot of the count pass's pairs (or null): when it
+// holds the workgroup's per-brick record counts, the ranges are reserved from
+// them and the rays are traced once; otherwise they are counted again here.
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom g,
                                                           const unsigned long long *offsets,
                                                           uint32_t *cursor, const uint32_t *mult,
-                                                          RecPtrs<T> rec) {
+                                                          RecPtrs<T> rec, const uint32_t *pairs) {
   __shared__ uint32_t slot[kHistMax];
   const bool lds = g.nbricks <= kHistMax;
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  if (pairs) pairs += (size_t)blockIdx.x * (kWgPairBytes / 4);
   if (lds) {
     for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) slot[k] = 0;
     __syncthreads();
-    trace_bricks<T, SUN>(
-        a, g, tile, [&](int b, const Ray<T> &, uint32_t, bool) { atomicAdd(&slot[b], 1u); },
-        [](uint32_t) {});
-    __syncthreads();
-    // reserve one contiguous range per brick for this workgroup's records
-    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x)
-      if (slot[k]) slot[k] = atomicAdd(&cursor[k], slot[k]);
+    const uint32_t np = pairs ? pairs[0] : kPairsOverflow;
+    if (np != kPairsOverflow) {
+      // reserve one contiguous range per brick for this workgroup's records
+      for (uint32_t q = threadIdx.x; q < np; q += blockDim.x) {
+        const uint32_t b = pairs[2 + 2 * q];
+        slot[b] = atomicAdd(&cursor[b], pairs[3 + 2 * q]);
+      }
+    } else {
+      trace_bricks<T, SUN>(
+          a, g, tile, [&](int b, const Ray<T> &, uint32_t, bool) { atomicAdd(&slot[b], 1u); },
+          [](uint32_t) {});
+      __syncthreads();
+      for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x)
+        if (slot[k]) slot[k] = atomicAdd(&cursor[k], slot[k]);
+    }
     __syncthreads();
   }
   unsigned long long pcur = 0;
@@ -541,6 +587,21 @@ size_t rec_bytes(size_t R, size_t rays) {
 
 size_t call_rays(const tg_call *c) { return (size_t)c->nv * c->n1 * c->n2; }
 
+// Bytes of the count pass's per-workgroup pairs for the plan's trace grid.
+template <typename T>
+size_t plan_scratch_bytes(const KArgs<T> &a) {
+  return (size_t)a.nv * (size_t)a.tiles_i * (size_t)a.tiles_j * kWgPairBytes;
+}
+
+// The same from the call alone (tg_bpj_plan_init): the trace grid of plan_count.
+size_t plan_scratch_bytes_for(const tg_call *c, const tg_bpj_plan *p) {
+  const int tj = p->trace_tile > 0 ? p->trace_tile : kPlanTraceTile;
+  if (tj <= 0 || tj > 256 || (tj & (tj - 1))) return 0;
+  const int bi = 256 / tj;
+  return (size_t)c->nv * (size_t)((c->n1 + bi - 1) / bi) * (size_t)((c->n2 + tj - 1) / tj) *
+         kWgPairBytes;
+}
+
 template <typename T>
 int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
   KArgs<T> a;
@@ -558,12 +619,20 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, s, counts, g.nbricks);
   static_assert(kPlanBlock == 256, "view-range launches assume 256-lane workgroups");
   const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
+  const uint64_t per_view = (uint64_t)a.tiles_i * (uint64_t)a.tiles_j;
+  uint32_t *pairs = (p->scratch && p->scratch_bytes >= plan_scratch_bytes(a)) ? (uint32_t *)p->scratch
+                                                                            : nullptr;
+  uint32_t *per_ray = p->ray_steps_out;
   rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
+    uint32_t *pl = pairs ? pairs + (size_t)av.t0 * per_view * (kWgPairBytes / 4) : nullptr;
     if (sun)
-      hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g, counts);
+      hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g, counts,
+                         pl, per_ray);
     else
-      hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g, counts);
+      hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
+                         counts, pl, per_ray);
   });
+  p->pairs_ready = pairs ? 1 : 0;
   if (rc) return rc;
   if ((rc = hip_check(hipGetLastError(), "plan count launch"))) return rc;
   std::vector<uint32_t> hc(g.nbricks);
@@ -647,13 +716,19 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, s, cursor, g.nbricks);
   RecPtrs<T> rp = rec_ptrs<T>(p);
   const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
+  // the count pass's per-workgroup pairs, when it wrote them into this scratch
+  const uint64_t per_view = (uint64_t)a.tiles_i * (uint64_t)a.tiles_j;
+  const uint32_t *pairs = (p->pairs_ready && p->scratch && p->scratch_bytes >= plan_scratch_bytes(a))
+                              ? (const uint32_t *)p->scratch
+                              : nullptr;
   rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
+    const uint32_t *pl = pairs ? pairs + (size_t)av.t0 * per_view * (kWgPairBytes / 4) : nullptr;
     if (sun)
       hipLaunchKernelGGL((k_plan_fill<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
-                         offsets, cursor, mult, rp);
+                         offsets, cursor, mult, rp, pl);
     else
       hipLaunchKernelGGL((k_plan_fill<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
-                         offsets, cursor, mult, rp);
+                         offsets, cursor, mult, rp, pl);
   });
   if (rc) return rc;
   if ((rc = hip_check(hipGetLastError(), "plan fill launch"))) return rc;
@@ -818,6 +893,8 @@ int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p) {
   p->nbricks = g.nbricks;
   for (int k = 0; k < 3; k++) p->nb_axis[k] = g.nb[k];
   p->meta_bytes = meta_layout(g.nbricks).total;
+  p->scratch_bytes = g.nbricks <= kHistMax ? plan_scratch_bytes_for(c, p) : 0;
+  p->pairs_ready = 0;
   p->records = 0;
   p->chunks = 0;
   p->rec_bytes = 0;

@@ -81,14 +81,17 @@ class Siddon:
     * ``counted_forward`` (default on): count each ray's steps once and run
       the projector's loop for that count;
     * ``tile_j``: pixels per row of a projector workgroup's ray tile (0: the
-      image's second axis rounded up to a power of two, at most 256).
+      image's second axis rounded up to a power of two, at most 256);
+    * ``plan_single_trace`` (default on): the plan's count pass hands its
+      per-workgroup record counts to the fill pass, which then traces every
+      ray once instead of twice.
     """
 
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
                  voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
                  tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0,
                  plan_order=2, plan_trace_tile=0, plan_tie_spread=None, acc64=None,
-                 counted_forward=True):
+                 counted_forward=True, plan_single_trace=True):
         self.data_header = list(data_header)
         self.map_header = dict(cube_header)
         check_headers(self.data_header, self.map_header)
@@ -142,6 +145,7 @@ class Siddon:
         self._plan_bufs = None
         self._plan_count = None
         self._auto_plan = None
+        self.plan_single_trace = bool(plan_single_trace)
         # float32 backprojections sum in a float64 accumulator (TG_ACC_F64), so
         # that the result does not depend on the order in which rays are added
         self.acc64 = (self.tdtype == torch.float32) if acc64 is None else bool(acc64)
@@ -152,12 +156,14 @@ class Siddon:
         self._xw = None  # voxel-weighted map read by forward projections
 
     # -- brick-privatised backprojection plan ------------------------------
-    # In "auto" mode the plan is used when the geometry has at least
-    # PLAN_MIN_STEPS steps (and PLAN_MIN_STEPS_PER_VOXEL steps per voxel of
-    # every time bin).  The brick kernel beats the atomic scatter even for
-    # sparse geometries (256^3 seen by 8 views of 64^2, 0.6 steps per voxel:
-    # 0.17 against 0.39 ms), and a build of a plan that small takes about 1 ms.
-    PLAN_MIN_STEPS_PER_VOXEL = 0
+    # In "auto" mode the plan is used when the geometry may take at least
+    # PLAN_MIN_STEPS steps: when rays x (n0 + n1 + n2), a bound on the step
+    # count that needs no pass over the rays, reaches it.  The brick kernel
+    # beats the atomic scatter even for sparse geometries (256^3 seen by 8
+    # views of 64^2, 0.6 steps per voxel: 0.17 against 0.39 ms), and a build
+    # of a plan that small takes about 1 ms.  The bound lets the plan's count
+    # pass write the per-ray step counts too, so that a geometry with a plan
+    # is traced twice in all (count, fill) before its first projection pair.
     PLAN_MIN_STEPS = 1 << 20
 
     def auto_tie_spread(self):
@@ -186,10 +192,8 @@ class Siddon:
         if self.n1 * self.n2 * self.nv >= 1 << 32:
             return False  # the plan's ray ids are 32-bit
         if self._auto_plan is None:
-            steps = self.count_steps()
-            voxels = int(np.prod(self.cube_shape)) * getattr(self, "nt", 1)
-            self._auto_plan = (steps >= self.PLAN_MIN_STEPS and
-                               steps >= self.PLAN_MIN_STEPS_PER_VOXEL * voxels)
+            bound = self.n1 * self.n2 * self.nv * sum(self.cube_shape)
+            self._auto_plan = bound >= self.PLAN_MIN_STEPS
         return self._auto_plan
 
     def _count_plan(self):
@@ -210,8 +214,20 @@ class Siddon:
             meta = torch.empty(max(int(p.meta_bytes), 1), dtype=torch.uint8, device=self.dev)
             p.meta = meta.data_ptr()
             p.meta_bytes = meta.numel()
+            # the count pass hands each trace workgroup's record counts to the
+            # fill pass through this scratch (dropped after the build), and
+            # writes the per-ray step counts the projector runs on
+            scratch = None
+            if int(p.scratch_bytes) > 0 and self.plan_single_trace:
+                scratch = torch.empty(int(p.scratch_bytes), dtype=torch.uint8, device=self.dev)
+                p.scratch = scratch.data_ptr()
+            if self._ray_steps is None:
+                self._ray_steps = torch.zeros(self.nv * self.n1 * self.n2, dtype=torch.int32,
+                                              device=self.dev)
+                p.ray_steps_out = self._ray_steps.data_ptr()
             N.check(lib.tg_bpj_plan_count(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
-            self._plan_count = (p, meta)
+            p.ray_steps_out = None
+            self._plan_count = (p, meta, scratch)
         return self._plan_count
 
     def plan_reserve_bytes(self):
@@ -221,8 +237,9 @@ class Siddon:
             return 0
         if self._plan_bufs is not None:
             return self.plan_bytes()
-        p, meta = self._count_plan()
-        return int(meta.numel()) + max(int(p.rec_bytes), 1)
+        p, meta, scratch = self._count_plan()
+        return (int(meta.numel()) + max(int(p.rec_bytes), 1)
+                + (int(scratch.numel()) if scratch is not None else 0))
 
     def build_plan(self):
         """Builds (once) the value-independent brick plan of this geometry."""
@@ -230,12 +247,18 @@ class Siddon:
             return self._plan
         lib = N.lib()
         c = self._call(N.TG_BPJ, self.xin, self.xout)
-        p, meta = self._count_plan()
+        p, meta, scratch = self._count_plan()
         self._plan_count = None
         rec = torch.empty(max(int(p.rec_bytes), 1), dtype=torch.uint8, device=self.dev)
         p.rec = rec.data_ptr()
         p.rec_bytes = rec.numel()
         N.check(lib.tg_bpj_plan_build(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
+        # the scratch is not needed once the fill has run; freeing it returns it to
+        # torch's allocator, which orders its reuse after this stream's work
+        p.scratch = None
+        p.scratch_bytes = 0
+        p.pairs_ready = 0
+        del scratch
         self._plan = p
         self._plan_bufs = (meta, rec)
         return p
@@ -307,6 +330,8 @@ class Siddon:
     def ray_steps(self):
         """Per-ray Siddon step counts of this geometry (device uint32, indexed
         (t * n1 + i) * n2 + j), computed once and reused by every projection."""
+        if self._ray_steps is None and self.use_plan() and self._plan is None:
+            self._count_plan()  # writes the per-ray counts as it counts the plan's records
         if self._ray_steps is None:
             c = self._call(N.TG_PJ, self.xin, self.xout, counted=False, apply_vox=False)
             per = torch.zeros(self.nv * self.n1 * self.n2, dtype=torch.int32, device=self.dev)

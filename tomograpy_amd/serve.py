@@ -64,6 +64,8 @@ class Request:
         self._cancel = False
         self.x = None
         self._pinned = False
+        # True: keep a device fingerprint of the solution (solution_fingerprint)
+        self.fingerprint = None
         self.iterations = 0
         self.residual = None
         self.t_admit = None
@@ -215,10 +217,24 @@ def _torch_dtype(dtype):
     return torch.float32 if np.dtype(dtype) == np.float32 else torch.float64
 
 
+def solution_fingerprint(x):
+    """Two int64 sums over the bit patterns of the solution x (device): of the
+    words, and of their squares (wrapping), computed on the device and left
+    there.  Equal solutions have equal fingerprints; a difference of one ulp
+    anywhere changes them.  Lets a caller compare many served and plain
+    solutions bit for bit without keeping or hashing them on the host."""
+    w = x.detach().contiguous().view(-1)
+    bits = w.view(torch.int32 if w.dtype == torch.float32 else torch.int64).to(torch.int64)
+    return torch.stack([bits.sum(), (bits * bits).sum()])
+
+
 def _download(req, x):
     """Starts the copy of the solution x (device) to the request's host buffer
     (pinned if the request was pinned, so that the copy does not block);
-    returns (host tensor, completion event)."""
+    returns (host tensor, completion event).  Takes the solution's fingerprint
+    first when the request asks for one (request.fingerprint = True)."""
+    if req.fingerprint is True:
+        req.fingerprint = solution_fingerprint(x)
     host = req.x_host
     if host is None or tuple(host.shape) != tuple(x.shape):
         host = torch.empty(tuple(x.shape), dtype=x.dtype, pin_memory=req._pinned)
@@ -247,8 +263,16 @@ class Server:
     """Continuous batching over CG reconstructions on one GPU."""
 
     def __init__(self, budget_bytes=64 << 30, max_active=64, check_every=5, tuner=None,
-                 order="shortest", speculative="auto"):
-        """order: "shortest" admits the waiting request with the least work first
+                 order="shortest", speculative="auto", step_work=None):
+        """step_work: the work (Request.work units) one engine step may take.  Each
+        step advances the admitted requests with the least work per iteration,
+        as many as fit in step_work (at least one); the others keep their state
+        on the device and wait.  Small requests are then batched together until
+        they fill the GPU, and a large request, which fills it alone, does not
+        hold them back: a request's iterations are as fast as its batch
+        allows.  None = STEP_WORK; float("inf") advances every admitted request
+        at every step.
+        order: "shortest" admits the waiting request with the least work first
         (traced steps per iteration, estimated from the geometry); "arrival" admits
         in arrival order.  speculative: read each residual check while the next
         step runs (a request ended by it has taken one more step, discarded: its
@@ -264,6 +288,7 @@ class Server:
         if speculative not in ("auto", True, False):
             raise ValueError("speculative must be True, False or 'auto'")
         self.speculative = speculative
+        self.step_work = self.STEP_WORK if step_work is None else float(step_work)
         self.budget = int(budget_bytes)
         self.max_active = int(max_active)
         self.check_every = int(check_every)
@@ -280,6 +305,20 @@ class Server:
     # Work (Request.work: rays x mean cube edge) of the active set under which
     # residual checks are read one step late: about 4 ms of GPU time per step.
     SPECULATE_WORK = 1e9
+    # Work of one engine step's batch: about 0.3 ms of GPU time, more than a
+    # batch needs to fill the 256 CUs with its projector and brick-plan launches.
+    STEP_WORK = 2.5e8
+
+    def _run_set(self, active):
+        """The admitted requests one engine step advances: the least work per
+        iteration first, as many as fit in step_work, at least one."""
+        out, w = [], 0.0
+        for a in sorted(active, key=lambda a: a[0].work()):
+            if out and w + a[0].work() > self.step_work:
+                break
+            out.append(a)
+            w += a[0].work()
+        return out
 
     def _read_scal(self):
         """Starts a copy of every slot's CG scalars to pinned host memory;
@@ -308,6 +347,7 @@ class Server:
         table = None
         t0 = time.perf_counter()
         steps = 0
+        run = []  # the requests the current table covers (see _run_set)
         checks = []  # residual checks not yet read (at most the last step's)
         ended = set()
         while pending or waiting or active or finishing or checks:
@@ -371,18 +411,19 @@ class Server:
                 if pending:
                     time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
                 continue
+            fresh = None
+            if active and table is None:
+                run = self._run_set(active)
+                table = CG.SegTable([s for _, s, _ in run], self.dev)
             spec = self.speculative
             if spec == "auto":
-                spec = sum(r.work() for r, _, _ in active) < self.SPECULATE_WORK
-            fresh = None
+                spec = bool(active) and sum(r.work() for r, _, _ in run) < self.SPECULATE_WORK
             if active:
-                if table is None:
-                    table = CG.SegTable([s for _, s, _ in active], self.dev)
-                CG.iterate([s for _, s, _ in active], table, self.tuner if len(active) > 1 else None)
+                CG.iterate([s for _, s, _ in run], table, self.tuner if len(run) > 1 else None)
                 steps += 1
                 if on_step is not None:
                     on_step(steps, requests)
-                due = [(r, s, row) for r, s, row in active
+                due = [(r, s, row) for r, s, row in run
                        if s.iterations % self.check_every == 0 or s.iterations >= r.maxiter]
                 if due:
                     # speculative: the solution at the check is kept in case the

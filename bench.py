@@ -482,8 +482,17 @@ def det_leg(args, reps=2):
     ch, hs, rng = W.config2_geometry(dtype, n=args.n, n_views=args.views, n_pix=args.pix)
     y = torch.from_numpy(rng.normal(size=(args.pix, args.pix, len(hs))).astype(dtype)).cuda()
     Pd = siddon_lo(hs, ch, dtype=dtype, deterministic=True)
-    ms = _event_ms(lambda: Pd.rmatvec(y), reps)
-    out = {"ms_backward": ms, "workspace_bytes": int(Pd._ws.numel()) if Pd._ws is not None else 0}
+    out = {"method": "canonical brick plan" if Pd.det_plan_ok() else "chunked passes"}
+    if Pd.det_plan_ok():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Pd.build_det_plan()
+        torch.cuda.synchronize()
+        out["plan_build_ms"] = (time.perf_counter() - t0) * 1e3
+        out["plan_records"] = int(Pd._det_plan.records)
+        out["plan_bytes"] = int(sum(b.numel() for b in Pd._det_plan_bufs))
+    out["ms_backward"] = _event_ms(lambda: Pd.rmatvec(y), reps)
+    out["workspace_bytes"] = int(Pd._ws.numel()) if Pd._ws is not None else 0
     del Pd
     torch.cuda.empty_cache()
     return out

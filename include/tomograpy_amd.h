@@ -151,6 +151,15 @@ int tg_count_steps(const tg_call *c, uint32_t *per_ray, unsigned long long *tota
  * of rays per pass.  Larger workspaces take more rows (views) per pass. */
 size_t tg_det_workspace_min_bytes(const tg_call *c);
 
+/* Workspace with which a TG_DETERMINISTIC backprojection of all views of *c
+ * builds a canonical brick plan inside it (each brick's records in ray order)
+ * and sums every brick's voxels in the reference's order through it, instead
+ * of the chunked passes; 0 when the geometry does not fit one (more than 2048
+ * bricks x time bins of 32 x 16 x 16 voxels).  An upper bound: the call checks
+ * the exact need after counting and takes the chunked path if it does not fit.
+ * No reference counterpart (the reference's order is its nthread = 1 run). */
+size_t tg_det_plan_workspace_bytes(const tg_call *c);
+
 /* Elementwise x[k] *= w[k] over a strided 3-D (or 4-D with nt bins) cube: the
  * voxel-weight half of the Thomson operator's adjoint (models.py:223-235). */
 int tg_scale_cube(const tg_call *c, void *stream);
@@ -185,7 +194,9 @@ typedef struct tg_bpj_plan {
   int32_t order;         /* 0 = records of a brick in trace order, 1 = permuted (spreads the
                             rays of a wavefront over the brick), 2 = trace order with each
                             pass of the backprojection workgroup (1024 records) sorted by
-                            decreasing step count (even walks within a wavefront) */
+                            decreasing step count (even walks within a wavefront),
+                            3 = canonical: ray order (view, row, column), for the
+                            deterministic backprojection */
   int32_t trace_tile;    /* pixels per row of the ray tiles the plan build traces (power of
                             two); sets which rays are neighbours in a brick's record list,
                             hence in a wavefront of the backprojection; 0 = 256 */
@@ -213,7 +224,13 @@ typedef struct tg_bpj_plan {
 } tg_bpj_plan;
 
 /* Fills the derived fields and meta_bytes for the geometry of *c (kind bpj or
- * bpjt; cube edges <= 1024). */
+ * bpjt; cube edges <= 1024).  Order 3 (canonical) plans keep each brick's
+ * records in ray order (view, row, column) for the deterministic
+ * backprojection: tg_bpj_plan_backproject with TG_DETERMINISTIC sums each
+ * voxel in that order, bit for bit the reference's nthread = 1 run.  They need
+ * the scratch (scratch_bytes, set here) through count and build, at most 2048
+ * bricks x time bins, bricks of at most 8192 voxels (default 32 x 16 x 16)
+ * and rows of 256 traced pixels. */
 int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p);
 /* Counts records per brick into p->meta; synchronises the stream; sets
  * records, chunks and rec_bytes. */

@@ -84,6 +84,7 @@ SIGNATURES = [
      [ctypes.POINTER(TgCall), _i32, _i32, _vp, _vp, _vp, _vp]),
     ("tg_count_steps", ctypes.c_int, [ctypes.POINTER(TgCall), _vp, _vp, _vp]),
     ("tg_det_workspace_min_bytes", _sz, [ctypes.POINTER(TgCall)]),
+    ("tg_det_plan_workspace_bytes", _sz, [ctypes.POINTER(TgCall)]),
     ("tg_scale_cube", ctypes.c_int, [ctypes.POINTER(TgCall), _vp]),
     ("tg_vox_apply", ctypes.c_int, [ctypes.POINTER(TgCall), _vp, ctypes.POINTER(_i64), _vp]),
     ("tg_regularize", ctypes.c_int, [_i32, ctypes.POINTER(_i32), _u32, _dbl, _i32, _vp, _vp, _vp,

@@ -390,6 +390,11 @@ int run_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, hipStream
   const bool bwd = (c->kind == TG_BPJ || c->kind == TG_BPJT);
   if (bwd && (c->flags & TG_DETERMINISTIC)) {
     if (ray_i >= 0) return set_error(TG_EINVAL, "deterministic mode applies to full/image calls");
+    if (t0 == 0 && t1 == c->nv) {
+      // all views: through a canonical plan when it fits the workspace
+      const int r = run_det_plan<T>(c, s);
+      if (r != kDetPlanUnfit) return r;
+    }
     return run_deterministic<T>(c, a, s);
   }
   const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;

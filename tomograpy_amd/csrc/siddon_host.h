@@ -57,6 +57,14 @@ int check_acc_jobs(const KArgs<T> *const *args, int njobs) {
 // (siddon.hip).
 int dispatch_siddon(const tg_call *c, int t0, int t1, int ray_i, int ray_j, void *stream);
 
+// Deterministic backprojection through a canonical plan built in the call's
+// workspace (siddon_plan.hip); kDetPlanUnfit when the geometry or workspace
+// does not allow it, and the chunked path below is taken instead.
+constexpr int kDetPlanUnfit = 1;
+template <typename T>
+int run_det_plan(const tg_call *c, hipStream_t s);
+size_t det_plan_workspace_bytes(const tg_call *c);
+
 // Deterministic backprojection (siddon_det.hip).
 template <typename T>
 int run_deterministic(const tg_call *c, const KArgs<T> &a, hipStream_t s);

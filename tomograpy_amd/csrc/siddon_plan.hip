@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,8 @@ struct BrickGeom {
   int32_t shuffle;    // records of a brick stored in a permuted order
   int32_t sort;       // records of each workgroup pass sorted by their step counts
   int32_t spread;     // low bits of the pass position reversed to order ties
+  int32_t canon;      // order 3: each brick's records in ray order (view, row, column),
+                      // cells unpadded, for the deterministic backprojection
 };
 
 // A record's ray id is the ray's linear index (t n1 + i) n2 + j: the index of
@@ -101,9 +104,13 @@ struct RecPtrs {
 // padded by one cell and each axis-1 plane by one more.  With power-of-two
 // pitches, voxels that differ only along axis 0 or 1 share an LDS bank, and
 // the neighbouring rays of a wavefront hit exactly such voxels together.
-__host__ __device__ __forceinline__ int brick_pitch1(const BrickGeom &g) { return g.b[2] + 1; }
+// A canonical plan (order 3) keeps the cells unpadded: its backprojection sums
+// each cell serially, so banks do not matter, and LDS is short.
+__host__ __device__ __forceinline__ int brick_pitch1(const BrickGeom &g) {
+  return g.canon ? g.b[2] : g.b[2] + 1;
+}
 __host__ __device__ __forceinline__ int brick_pitch0(const BrickGeom &g) {
-  return g.b[1] * (g.b[2] + 1) + 1;
+  return g.canon ? g.b[1] * g.b[2] : g.b[1] * (g.b[2] + 1) + 1;
 }
 __host__ __device__ __forceinline__ long long brick_lds_cells(const BrickGeom &g) {
   return (long long)g.b[0] * brick_pitch0(g);
@@ -187,10 +194,12 @@ constexpr size_t kWgPairBytes = 8 * (size_t)(kWgPairs + 1);
 
 // pairs: the workgroup's kWgPairBytes slot of per-workgroup pairs (or null);
 // per_ray: per-ray step counts out (or null).
+// rowmat (canonical plans, or null): this launch's rows of the tile x brick
+// matrix of record counts; the workgroup writes its tile's row.
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom g,
                                                            uint32_t *counts, uint32_t *pairs,
-                                                           uint32_t *per_ray) {
+                                                           uint32_t *per_ray, uint32_t *rowmat) {
   __shared__ uint32_t hist[kHistMax];
   __shared__ uint32_t npairs;
   const bool lds = g.nbricks <= kHistMax;
@@ -224,9 +233,112 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
     }
     __syncthreads();
     if (pairs && threadIdx.x == 0) pairs[0] = npairs <= (uint32_t)kWgPairs ? npairs : kPairsOverflow;
+    if (rowmat) {
+      uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
+      for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) row[k] = hist[k];
+    }
   } else if (pairs && threadIdx.x == 0) {
     pairs[0] = kPairsOverflow;
   }
+}
+
+// ---- canonical plans (order 3): every brick's records in ray order ----------
+// The count pass leaves a matrix of record counts, one row per trace tile
+// (256 pixels of one image row: tiles in (view, row, column) order) and one
+// column per brick.  An exclusive scan down each column gives every tile the
+// position of its first record in each brick, and inside a tile a lane's
+// record goes after those of the lanes (pixels) before it that enter the same
+// brick: per-brick lane masks in LDS.  The brick's records are then in the
+// order of their ray ids, the order in which the reference's single-threaded
+// run adds into the brick's voxels (C:489-494 with nthread = 1).
+constexpr int kDetMaxBricks = 2048;  // 256-bit lane masks of all bricks in LDS (64 KiB)
+constexpr int kDetSegRows = 256;     // rows per segment of the column scan
+
+// Segment sums of the columns: block (s, cb) sums rows [s * 256, (s + 1) * 256)
+// of columns [cb * 256, cb * 256 + 256) into seg[s * nb + col].
+__global__ __launch_bounds__(256) void k_det_colsum(const uint32_t *__restrict__ mat, int64_t rows,
+                                                    int nb, uint32_t *__restrict__ seg) {
+  const int col = blockIdx.y * 256 + threadIdx.x;
+  if (col >= nb) return;
+  const int64_t r0 = (int64_t)blockIdx.x * kDetSegRows;
+  const int64_t r1 = r0 + kDetSegRows < rows ? r0 + kDetSegRows : rows;
+  uint32_t sum = 0;
+  for (int64_t r = r0; r < r1; r++) sum += mat[r * nb + col];
+  seg[(int64_t)blockIdx.x * nb + col] = sum;
+}
+
+// Exclusive scan of each column's segment sums (one thread per column).
+__global__ __launch_bounds__(256) void k_det_segscan(uint32_t *__restrict__ seg, int64_t nseg,
+                                                     int nb) {
+  const int col = blockIdx.x * 256 + threadIdx.x;
+  if (col >= nb) return;
+  uint32_t run = 0;
+  for (int64_t q = 0; q < nseg; q++) {
+    const uint32_t v = seg[q * nb + col];
+    seg[q * nb + col] = run;
+    run += v;
+  }
+}
+
+// mat[r][col] = records of brick col from the tiles before r (exclusive scan).
+__global__ __launch_bounds__(256) void k_det_colapply(uint32_t *__restrict__ mat, int64_t rows,
+                                                      int nb, const uint32_t *__restrict__ seg) {
+  const int col = blockIdx.y * 256 + threadIdx.x;
+  if (col >= nb) return;
+  const int64_t r0 = (int64_t)blockIdx.x * kDetSegRows;
+  const int64_t r1 = r0 + kDetSegRows < rows ? r0 + kDetSegRows : rows;
+  uint32_t run = seg[(int64_t)blockIdx.x * nb + col];
+  for (int64_t r = r0; r < r1; r++) {
+    const uint32_t v = mat[r * nb + col];
+    mat[r * nb + col] = run;
+    run += v;
+  }
+}
+
+// Fill of a canonical plan: two traces per ray, the first to set each
+// brick's lane mask, the second to write each record at the brick's offset +
+// the tile's scanned count + the number of earlier lanes in the mask.
+template <typename T, bool SUN>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, BrickGeom g,
+                                                                const unsigned long long *offsets,
+                                                                const uint32_t *rowmat,
+                                                                RecPtrs<T> rec) {
+  __shared__ uint32_t mask[kDetMaxBricks * (kPlanBlock / 32)];
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int words = g.nbricks * (kPlanBlock / 32);
+  for (int k = threadIdx.x; k < words; k += blockDim.x) mask[k] = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x;
+  const uint32_t bit = 1u << (lane & 31);
+  trace_bricks<T, SUN>(
+      a, g, tile,
+      [&](int b, const Ray<T> &, uint32_t, bool) { atomicOr(&mask[b * 8 + (lane >> 5)], bit); },
+      [](uint32_t) {});
+  __syncthreads();
+  const uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
+  unsigned long long pcur = 0;
+  uint32_t lcur = 0;
+  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &r, uint32_t rid, bool first) {
+    if (first) {
+      RayEntry<T> e;
+      e.pabs[0] = r.pabs[0];
+      e.pabs[1] = r.pabs[1];
+      e.pabs[2] = r.pabs[2];
+      e.up = (r.up[0] + 1) | ((r.up[1] + 1) << 2) | ((r.up[2] + 1) << 4);
+      rec.ray[rid] = e;
+    }
+    const uint32_t *m = &mask[b * 8];
+    uint32_t rank = 0;
+    for (uint32_t w = 0; w < (lane >> 5); w++) rank += __popc(m[w]);
+    rank += __popc(m[lane >> 5] & (bit - 1u));
+    const unsigned long long p = offsets[b] + row[b] + rank;
+    rec.rid[p] = rid;
+    rec.d0[p] = r.D[0];
+    rec.d1[p] = r.D[1];
+    rec.d2[p] = r.D[2];
+    pcur = p;
+    lcur = brick_local(g, r.iv);
+  }, [&](uint32_t steps) { rec.lq[pcur] = lcur | (steps << 16); });
 }
 
 // pairs: this workgroup's slot of the count pass's pairs (or null): when it
@@ -463,6 +575,188 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
   }
 }
 
+// ---- deterministic backprojection through a canonical plan ---------------
+// One workgroup per brick walks the brick's records in ray order, in batches
+// of up to kBpjBlock records (one per lane) whose steps fit `mcap`
+// contributions.  Per batch: every lane writes its contributions d * w (in T,
+// as C:923-928 forms them) at consecutive slots, numbered in (ray, step)
+// order by a scan of the step counts; the slots are bucketed by cell
+// (counting sort in LDS); and one lane per cell adds its bucket, sorted back
+// into slot order, to the cell's running sum in T.  Each voxel therefore adds
+// its contributions one at a time in the reference's single-thread order
+// (view, row, column, C:489-494), starting from the cube's value, and the
+// result is bit-identical to that run.
+
+// Inclusive scan of v over the workgroup (kBpjBlock lanes); wsum holds
+// kBpjBlock / 64 + 1 words of LDS.
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *wsum) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o, 64);
+    if (lane >= (uint32_t)o) v += u;
+  }
+  if (lane == 63u) wsum[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t w = 0; w < kBpjBlock / 64; w++) {
+      const uint32_t x = wsum[w];
+      wsum[w] = run;
+      run += x;
+    }
+    wsum[kBpjBlock / 64] = run;
+  }
+  __syncthreads();
+  return v + wsum[wave];
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ size_t det_lds_bytes(int cells, int mcap) {
+  return (size_t)cells * sizeof(T) + (size_t)mcap * sizeof(T) + (size_t)cells * 4 +
+         4 * (kBpjBlock / 64 + 4) + (size_t)mcap * 4;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeom g,
+                                                            const unsigned long long *offsets,
+                                                            const uint32_t *border,
+                                                            RecPtrs<T> rec, int mcap) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int cells = g.b[0] * g.b[1] * g.b[2];
+  T *acc = reinterpret_cast<T *>(smem_raw);
+  T *uval = acc + cells;
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(uval + mcap);
+  uint32_t *wsum = cnt + cells;  // kBpjBlock / 64 + 1 words, then the batch's record and slot counts
+  uint32_t *batch = wsum + (kBpjBlock / 64 + 1);
+  uint16_t *ucell = reinterpret_cast<uint16_t *>(batch + 3);
+  uint16_t *sidx = ucell + mcap;
+  const int b = (int)border[blockIdx.x];
+  if (offsets[b] == offsets[b + 1]) return;  // no records: the brick is left as it is
+  const int nsp = g.nb[0] * g.nb[1] * g.nb[2];
+  const int bin = b / nsp, bs = b % nsp;
+  const int bz = bs % g.nb[2], by = (bs / g.nb[2]) % g.nb[1], bx = bs / (g.nb[2] * g.nb[1]);
+  const int lo0 = bx << g.lg[0], lo1 = by << g.lg[1], lo2 = bz << g.lg[2];
+  const int64_t boff = (int64_t)bin * a.cs[3];
+  const uint32_t tid = threadIdx.x;
+  // the brick's voxels as they are: the sums continue from them (in place)
+  for (int c = tid; c < cells; c += kBpjBlock) {
+    const int l2 = c & (g.b[2] - 1), l1 = (c >> g.lg[2]) & (g.b[1] - 1), l0 = c >> (g.lg[2] + g.lg[1]);
+    const int64_t i0 = lo0 + l0, i1 = lo1 + l1, i2 = lo2 + l2;
+    const bool inside = i0 < (int64_t)a.n[0] && i1 < (int64_t)a.n[1] && i2 < (int64_t)a.n[2];
+    acc[c] = inside ? a.cube[boff + i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2]] : (T)0;
+  }
+  const bool view_major = a.ds[1] == 1 && a.ds[0] == (int64_t)a.n2 &&
+                          a.ds[2] == (int64_t)a.n1 * (int64_t)a.n2;
+  const int dm0 = brick_pitch0(g), dm1 = brick_pitch1(g);
+  unsigned long long r = offsets[b];
+  const unsigned long long r1 = offsets[b + 1];
+  __syncthreads();
+  while (r < r1) {
+    const unsigned long long p = r + tid;
+    const bool valid = p < r1;
+    uint32_t lq = 0, n = 0;
+    T D0 = 0, D1 = 0, D2 = 0, w = 0;
+    RayEntry<T> e;
+    e.pabs[0] = e.pabs[1] = e.pabs[2] = (T)0;
+    e.up = 0;
+    if (valid) {
+      const uint32_t rid = rec.rid[p];
+      lq = rec.lq[p];
+      D0 = rec.d0[p];
+      D1 = rec.d1[p];
+      D2 = rec.d2[p];
+      int64_t did = rid;
+      if (!view_major) {
+        int t, i, j;
+        ray_pixel((uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
+        did = i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+      }
+      w = a.data[did];
+      if (a.pixw) w = a.pixw[did] * w;
+      e = rec.ray[rid];
+      // a zero pixel adds zeros: its steps are left out
+      n = w != (T)0 ? lq >> 16 : 0u;
+    }
+    if (tid == 0) {
+      batch[0] = 0;
+      batch[1] = 0;
+    }
+    const uint32_t incl = block_incl_scan(n, wsum);  // synchronises: batch[] is zero
+    const bool take = valid && incl <= (uint32_t)mcap;
+    if (take) {
+      atomicAdd(&batch[0], 1u);
+      atomicMax(&batch[1], incl);
+    }
+    for (int c = tid; c < cells; c += kBpjBlock) cnt[c] = 0;
+    __syncthreads();
+    const uint32_t nrec = batch[0], nslot = batch[1];
+    if (take && n) {
+      const int u0 = ((e.up >> 0) & 3) - 1, u1 = ((e.up >> 2) & 3) - 1, u2 = ((e.up >> 4) & 3) - 1;
+      int cell = (int)(lq & 0xffffu);
+      uint32_t m = incl - n;
+      for (uint32_t k = 0; k < n; k++, m++) {
+        const T d = tmin(tmin(D0, D1), D2);
+        const bool sx = (D0 == d), sy = (D1 == d), sz = (D2 == d);
+        D0 = sx ? e.pabs[0] : D0 - d;
+        D1 = sy ? e.pabs[1] : D1 - d;
+        D2 = sz ? e.pabs[2] : D2 - d;
+        ucell[m] = (uint16_t)cell;
+        uval[m] = d * w;
+        atomicAdd(&cnt[cell], 1u);
+        cell += (sx ? u0 * dm0 : 0) + (sy ? u1 * dm1 : 0) + (sz ? u2 : 0);
+      }
+    }
+    __syncthreads();
+    // bucket starts: cnt[c] = (exclusive prefix << 16) | count
+    const int per = (cells + kBpjBlock - 1) / kBpjBlock;
+    const int c0 = (int)tid * per;
+    uint32_t local = 0;
+    for (int c = c0; c < c0 + per && c < cells; c++) local += cnt[c];
+    uint32_t run = block_incl_scan(local, wsum) - local;
+    for (int c = c0; c < c0 + per && c < cells; c++) {
+      const uint32_t k = cnt[c];
+      cnt[c] = (run << 16) | k;
+      run += k;
+    }
+    __syncthreads();
+    for (uint32_t m = tid; m < nslot; m += kBpjBlock) {
+      const uint32_t c = ucell[m];
+      const uint32_t pos = atomicAdd(&cnt[c], 1u << 16) >> 16;
+      sidx[pos] = (uint16_t)m;
+    }
+    __syncthreads();
+    // each cell adds its bucket in slot order, one contribution at a time
+    for (int c = tid; c < cells; c += kBpjBlock) {
+      const uint32_t v = cnt[c];
+      const uint32_t k = v & 0xffffu;
+      if (!k) continue;
+      const uint32_t end = v >> 16, st = end - k;
+      for (uint32_t q = st + 1; q < end; q++) {  // insertion sort: buckets are short
+        const uint16_t x = sidx[q];
+        uint32_t z = q;
+        while (z > st && sidx[z - 1] > x) {
+          sidx[z] = sidx[z - 1];
+          z--;
+        }
+        sidx[z] = x;
+      }
+      T sum = acc[c];
+      for (uint32_t q = st; q < end; q++) sum = sum + uval[sidx[q]];
+      acc[c] = sum;
+    }
+    __syncthreads();
+    r += nrec;
+  }
+  for (int c = tid; c < cells; c += kBpjBlock) {
+    const int l2 = c & (g.b[2] - 1), l1 = (c >> g.lg[2]) & (g.b[1] - 1), l0 = c >> (g.lg[2] + g.lg[1]);
+    const int64_t i0 = lo0 + l0, i1 = lo1 + l1, i2 = lo2 + l2;
+    if (i0 >= (int64_t)a.n[0] || i1 >= (int64_t)a.n[1] || i2 >= (int64_t)a.n[2]) continue;
+    if (a.voxmask && a.voxmask[i0 * a.vs[0] + i1 * a.vs[1] + i2 * a.vs[2]]) continue;
+    a.cube[boff + i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2]] = acc[c];
+  }
+}
+
 // One workgroup per chunk of one brick's records.
 template <typename T>
 __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj(KArgs<T> a, BrickGeom g,
@@ -534,9 +828,22 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
   g.chunk = p->chunk_records > 0 ? p->chunk_records : 32768;
   g.shuffle = p->order == 1 ? 1 : 0;
   g.sort = p->order == 2 ? 1 : 0;
+  g.canon = p->order == 3 ? 1 : 0;
+  if (p->order < 0 || p->order > 3) return set_error(TG_EINVAL, "plan order %d outside [0, 3]", p->order);
   if (p->tie_spread < 0 || p->tie_spread > 10)
     return set_error(TG_EINVAL, "tie_spread %d outside [0, 10]", p->tie_spread);
   g.spread = p->tie_spread;
+  if (g.canon) {
+    // lane masks of every brick in LDS; trace tiles of 256 pixels of one row;
+    // cells and slots numbered in 16 bits
+    if (g.nbricks > kDetMaxBricks)
+      return set_error(TG_EINVAL, "a canonical plan holds at most %d bricks x time bins, not %d",
+                       kDetMaxBricks, g.nbricks);
+    if (p->trace_tile != 0 && p->trace_tile != kPlanTraceTile)
+      return set_error(TG_EINVAL, "a canonical plan traces rows of %d pixels", kPlanTraceTile);
+    if (g.b[0] * g.b[1] * g.b[2] > 8192)
+      return set_error(TG_EINVAL, "a canonical plan's bricks hold at most 8192 voxels");
+  }
   return TG_OK;
 }
 
@@ -545,9 +852,11 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
 constexpr unsigned long long kAutoChunks = 1024;
 
 struct MetaLayout {
-  size_t counts, offsets, cursor, chunk_prefix, mult, total;
+  size_t counts, offsets, cursor, chunk_prefix, mult, border, total;
 };
 
+// border: the bricks in decreasing order of their record counts (the launch
+// order of the deterministic backprojection, longest first).
 MetaLayout meta_layout(int nbricks) {
   MetaLayout m;
   m.counts = 0;
@@ -555,8 +864,26 @@ MetaLayout meta_layout(int nbricks) {
   m.cursor = align_up(m.offsets + 8 * ((size_t)nbricks + 1), 256);
   m.chunk_prefix = align_up(m.cursor + 4 * (size_t)nbricks, 256);
   m.mult = align_up(m.chunk_prefix + 8 * ((size_t)nbricks + 1), 256);
-  m.total = align_up(m.mult + 4 * (size_t)nbricks, 256);
+  m.border = align_up(m.mult + 4 * (size_t)nbricks, 256);
+  m.total = align_up(m.border + 4 * (size_t)nbricks, 256);
   return m;
+}
+
+// Scratch of a canonical plan: the tile x brick count matrix and the segment
+// sums of its column scan.
+struct CanonScratch {
+  size_t mat, seg, total;
+  int64_t rows, nseg;
+};
+
+CanonScratch canon_scratch(int64_t tiles, int nbricks) {
+  CanonScratch c;
+  c.rows = tiles;
+  c.nseg = (tiles + kDetSegRows - 1) / kDetSegRows;
+  c.mat = 0;
+  c.seg = align_up(4 * (size_t)tiles * (size_t)nbricks, 256);
+  c.total = align_up(c.seg + 4 * (size_t)c.nseg * (size_t)nbricks, 256);
+  return c;
 }
 
 template <typename T>
@@ -620,19 +947,25 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
   static_assert(kPlanBlock == 256, "view-range launches assume 256-lane workgroups");
   const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
   const uint64_t per_view = (uint64_t)a.tiles_i * (uint64_t)a.tiles_j;
-  uint32_t *pairs = (p->scratch && p->scratch_bytes >= plan_scratch_bytes(a)) ? (uint32_t *)p->scratch
-                                                                            : nullptr;
+  const CanonScratch cs = canon_scratch((int64_t)c->nv * (int64_t)per_view, g.nbricks);
+  if (g.canon && (!p->scratch || p->scratch_bytes < cs.total))
+    return set_error(TG_EINVAL, "a canonical plan needs a scratch of %zu bytes", cs.total);
+  uint32_t *pairs = (!g.canon && p->scratch && p->scratch_bytes >= plan_scratch_bytes(a))
+                        ? (uint32_t *)p->scratch
+                        : nullptr;
+  uint32_t *rowmat = g.canon ? (uint32_t *)((char *)p->scratch + cs.mat) : nullptr;
   uint32_t *per_ray = p->ray_steps_out;
   rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
     uint32_t *pl = pairs ? pairs + (size_t)av.t0 * per_view * (kWgPairBytes / 4) : nullptr;
+    uint32_t *rm = rowmat ? rowmat + (size_t)av.t0 * per_view * (size_t)g.nbricks : nullptr;
     if (sun)
       hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g, counts,
-                         pl, per_ray);
+                         pl, per_ray, rm);
     else
       hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
-                         counts, pl, per_ray);
+                         counts, pl, per_ray, rm);
   });
-  p->pairs_ready = pairs ? 1 : 0;
+  p->pairs_ready = (pairs || rowmat) ? 1 : 0;
   if (rc) return rc;
   if ((rc = hip_check(hipGetLastError(), "plan count launch"))) return rc;
   std::vector<uint32_t> hc(g.nbricks);
@@ -688,6 +1021,24 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
   if ((rc = hip_check(hipMemcpyAsync(meta + m.mult, mult.data(), 4 * mult.size(),
                                      hipMemcpyHostToDevice, s), "plan multiplier copy")))
     return rc;
+  // bricks with records first, most records first (stable: ties by index)
+  std::vector<uint32_t> border(g.nbricks);
+  for (int b = 0; b < g.nbricks; b++) border[b] = (uint32_t)b;
+  std::stable_sort(border.begin(), border.end(),
+                   [&](uint32_t x, uint32_t y) { return hc[x] > hc[y]; });
+  if ((rc = hip_check(hipMemcpyAsync(meta + m.border, border.data(), 4 * border.size(),
+                                     hipMemcpyHostToDevice, s), "plan brick order copy")))
+    return rc;
+  if (g.canon && cs.rows > 0) {
+    // column scan of the count matrix: each tile's first record in each brick
+    uint32_t *seg = (uint32_t *)((char *)p->scratch + cs.seg);
+    const dim3 grid2((uint32_t)cs.nseg, (uint32_t)((g.nbricks + 255) / 256));
+    hipLaunchKernelGGL(k_det_colsum, grid2, dim3(256), 0, s, rowmat, cs.rows, g.nbricks, seg);
+    hipLaunchKernelGGL(k_det_segscan, dim3((uint32_t)((g.nbricks + 255) / 256)), dim3(256), 0, s,
+                       seg, cs.nseg, g.nbricks);
+    hipLaunchKernelGGL(k_det_colapply, grid2, dim3(256), 0, s, rowmat, cs.rows, g.nbricks, seg);
+    if ((rc = hip_check(hipGetLastError(), "canonical plan scan launch"))) return rc;
+  }
   if ((rc = hip_check(hipStreamSynchronize(s), "plan meta sync"))) return rc;
   p->records = (int64_t)tot;
   p->chunks = (int64_t)ch;
@@ -718,6 +1069,24 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   const bool sun = (c->flags & TG_OBSTACLE_SUN) != 0;
   // the count pass's per-workgroup pairs, when it wrote them into this scratch
   const uint64_t per_view = (uint64_t)a.tiles_i * (uint64_t)a.tiles_j;
+  if (g.canon) {
+    // ray order inside every brick, from the count pass's scanned count matrix
+    const CanonScratch cs = canon_scratch((int64_t)c->nv * (int64_t)per_view, g.nbricks);
+    if (!p->pairs_ready || !p->scratch || p->scratch_bytes < cs.total)
+      return set_error(TG_EINVAL, "a canonical plan's build needs the scratch its count filled");
+    const uint32_t *rowmat = (const uint32_t *)((const char *)p->scratch + cs.mat);
+    rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
+      const uint32_t *rm = rowmat + (size_t)av.t0 * per_view * (size_t)g.nbricks;
+      if (sun)
+        hipLaunchKernelGGL((k_plan_fill_canon<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av,
+                           g, offsets, rm, rp);
+      else
+        hipLaunchKernelGGL((k_plan_fill_canon<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av,
+                           g, offsets, rm, rp);
+    });
+    if (rc) return rc;
+    return hip_check(hipGetLastError(), "canonical plan fill launch");
+  }
   const uint32_t *pairs = (p->pairs_ready && p->scratch && p->scratch_bytes >= plan_scratch_bytes(a))
                               ? (const uint32_t *)p->scratch
                               : nullptr;
@@ -765,6 +1134,35 @@ int launch_plan_bpj(const KArgs<T> &a, const BrickGeom &g, const tg_bpj_plan *p,
   return hip_check(hipGetLastError(), "plan bpj launch");
 }
 
+// Slots of contributions per batch of the deterministic kernel: the LDS left
+// by the brick's running sums and bucket words.
+template <typename T>
+int det_slots(int cells) {
+  int mcap = 8192;
+  while (mcap > 256 && det_lds_bytes<T>(cells, mcap) > (size_t)kPlanLdsBytes) mcap >>= 1;
+  return mcap;
+}
+
+template <typename T>
+int launch_plan_bpj_det(const KArgs<T> &a, const BrickGeom &g, const tg_bpj_plan *p,
+                        hipStream_t s) {
+  if (p->records == 0) return TG_OK;
+  const int cells = g.b[0] * g.b[1] * g.b[2];
+  const int mcap = det_slots<T>(cells);
+  const size_t lds = det_lds_bytes<T>(cells, mcap);
+  if (lds > (size_t)kPlanLdsBytes)
+    return set_error(TG_EINVAL, "a brick of %d voxels leaves no LDS for the deterministic sums", cells);
+  static bool attr_set = false;
+  int rc = allow_full_lds(k_plan_bpj_det<T>, attr_set);
+  if (rc) return rc;
+  const MetaLayout m = meta_layout(g.nbricks);
+  const char *meta = (const char *)p->meta;
+  hipLaunchKernelGGL((k_plan_bpj_det<T>), dim3((uint32_t)g.nbricks), dim3(kBpjBlock), lds, s, a, g,
+                     (const unsigned long long *)(meta + m.offsets),
+                     (const uint32_t *)(meta + m.border), rec_ptrs<T>(p), mcap);
+  return hip_check(hipGetLastError(), "deterministic plan bpj launch");
+}
+
 template <typename T>
 int plan_bpj(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   if (c->kind != TG_BPJ && c->kind != TG_BPJT)
@@ -779,6 +1177,11 @@ int plan_bpj(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   if (!p->meta || p->meta_bytes < meta_layout(g.nbricks).total || !p->rec ||
       p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
     return set_error(TG_EINVAL, "plan buffers missing or too small");
+  if (c->flags & TG_DETERMINISTIC) {
+    if (!g.canon)
+      return set_error(TG_EINVAL, "deterministic plan backprojection needs a plan of order 3");
+    return launch_plan_bpj_det<T>(a, g, p, s);
+  }
   const size_t lds = sizeof(Acc) * (size_t)brick_lds_cells(g);
   if (lds > (size_t)kPlanLdsBytes) return set_error(TG_EINVAL, "brick of %zu bytes exceeds LDS", lds);
   if (p->chunks == 0) return TG_OK;
@@ -798,6 +1201,8 @@ template <typename T>
 int plan_job(const tg_call *c, const tg_bpj_plan *p, PlanJob<T> &j, size_t &lds) {
   if (c->kind != TG_BPJ && c->kind != TG_BPJT)
     return set_error(TG_EINVAL, "plan backprojection needs kind bpj or bpjt");
+  if (c->flags & TG_DETERMINISTIC)
+    return set_error(TG_EINVAL, "deterministic backprojection is not batched");
   int rc = make_args<T>(c, 0, c->nv, j.a);
   if (rc) return rc;
   if (!c->cube || !c->data) return set_error(TG_EINVAL, "cube and data pointers are required");
@@ -875,6 +1280,12 @@ extern "C" {
 
 int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p) {
   if (!c || !p) return set_error(TG_EINVAL, "null argument");
+  if (p->order == 3 && p->brick[0] == 0 && p->brick[1] == 0 && p->brick[2] == 0) {
+    // canonical plans: 8192-voxel bricks, long across the image rows' rays
+    p->brick[0] = 32;
+    p->brick[1] = 16;
+    p->brick[2] = 16;
+  }
   if (p->brick[0] == 0 && p->brick[1] == 0 && p->brick[2] == 0) {
     // default: 32 x 16 x 32 voxels = 128 KiB of double accumulators; for cubes
     // of edge <= 96, 16 x 16 x 32 (64 KiB, two workgroups per CU), whose
@@ -893,7 +1304,12 @@ int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p) {
   p->nbricks = g.nbricks;
   for (int k = 0; k < 3; k++) p->nb_axis[k] = g.nb[k];
   p->meta_bytes = meta_layout(g.nbricks).total;
-  p->scratch_bytes = g.nbricks <= kHistMax ? plan_scratch_bytes_for(c, p) : 0;
+  if (g.canon) {
+    const size_t per_tile = plan_scratch_bytes_for(c, p) / kWgPairBytes;
+    p->scratch_bytes = canon_scratch((int64_t)per_tile, g.nbricks).total;
+  } else {
+    p->scratch_bytes = g.nbricks <= kHistMax ? plan_scratch_bytes_for(c, p) : 0;
+  }
   p->pairs_ready = 0;
   p->records = 0;
   p->chunks = 0;
@@ -940,4 +1356,64 @@ int tg_bpj_plan_batch(const tg_call *calls, const tg_bpj_plan *plans, int32_t nj
   return set_error(TG_EINVAL, "bad dtype %d", calls[0].dtype);
 }
 
+size_t tg_det_plan_workspace_bytes(const tg_call *c) {
+  if (!c) return 0;
+  return tg::det_plan_workspace_bytes(c);
+}
+
 }  // extern "C"
+
+namespace tg {
+
+// The deterministic backprojection of a full call (all views) through a
+// canonical plan built inside the caller's workspace: meta, count matrix and
+// records carved from it.  Returns TG_OK when done, kDetPlanUnfit when the
+// geometry or the workspace does not allow it (nothing has been changed in
+// the cube then), or an error.
+template <typename T>
+int run_det_plan(const tg_call *c, hipStream_t s) {
+  tg_call cb = *c;
+  cb.flags &= ~TG_DETERMINISTIC;
+  cb.acc = nullptr;
+  tg_bpj_plan p;
+  memset(&p, 0, sizeof(p));
+  p.order = 3;
+  if (tg_bpj_plan_init(&cb, &p) != TG_OK) return kDetPlanUnfit;
+  char *ws = (char *)c->workspace;
+  const size_t meta = align_up(p.meta_bytes, 256), scratch = align_up(p.scratch_bytes, 256);
+  if (!ws || c->workspace_bytes < meta + scratch + 256) return kDetPlanUnfit;
+  p.meta = ws;
+  p.scratch = ws + meta;
+  int rc = plan_count<T>(&cb, &p, s);
+  if (rc) return rc;
+  const size_t rec = rec_bytes<T>((size_t)p.records, call_rays(c));
+  if (c->workspace_bytes < meta + scratch + rec) return kDetPlanUnfit;
+  p.rec = ws + meta + scratch;
+  p.rec_bytes = rec;
+  if ((rc = plan_build<T>(&cb, &p, s))) return rc;
+  return plan_bpj<T>(c, &p, s);
+}
+
+template int run_det_plan<float>(const tg_call *, hipStream_t);
+template int run_det_plan<double>(const tg_call *, hipStream_t);
+
+// Workspace bytes of run_det_plan for *c: meta, count matrix and an upper
+// bound of the records (every traced ray entering every brick on its way is
+// bounded by n0 / b0 + n1 / b1 + n2 / b2 + 3 bricks per ray); 0 when the
+// geometry does not fit a canonical plan.
+size_t det_plan_workspace_bytes(const tg_call *c) {
+  tg_call cb = *c;
+  cb.flags &= ~TG_DETERMINISTIC;
+  tg_bpj_plan p;
+  memset(&p, 0, sizeof(p));
+  p.order = 3;
+  if (tg_bpj_plan_init(&cb, &p) != TG_OK) return 0;
+  const size_t rays = call_rays(c);
+  const size_t per_ray = (size_t)(c->map.n[0] / p.brick[0] + c->map.n[1] / p.brick[1] +
+                                  c->map.n[2] / p.brick[2] + 3);
+  const size_t rec = c->dtype == TG_F64 ? rec_bytes<double>(rays * per_ray, rays)
+                                        : rec_bytes<float>(rays * per_ray, rays);
+  return align_up(p.meta_bytes, 256) + align_up(p.scratch_bytes, 256) + rec;
+}
+
+}  // namespace tg

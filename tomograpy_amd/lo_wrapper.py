@@ -146,6 +146,10 @@ class Siddon:
         self._plan_count = None
         self._auto_plan = None
         self.plan_single_trace = bool(plan_single_trace)
+        self.use_det_plan = True  # deterministic mode through a canonical plan when it fits
+        self._det_ok = None
+        self._det_plan = None
+        self._det_plan_bufs = None
         # float32 backprojections sum in a float64 accumulator (TG_ACC_F64), so
         # that the result does not depend on the order in which rays are added
         self.acc64 = (self.tdtype == torch.float32) if acc64 is None else bool(acc64)
@@ -266,6 +270,65 @@ class Siddon:
     def plan_bytes(self):
         return 0 if self._plan_bufs is None else sum(b.numel() for b in self._plan_bufs)
 
+    # -- deterministic backprojection through a canonical plan -----------------
+    def det_plan_ok(self):
+        """Whether the deterministic backprojection can use a canonical plan (its
+        records in ray order inside each brick): the geometry's bricks (x time
+        bins) must fit one canonical plan, and its count matrix (trace tiles x
+        bricks) DET_PLAN_MAX_SCRATCH bytes.  Otherwise the unplanned
+        deterministic mode runs."""
+        if self._det_ok is None:
+            self._det_ok = False
+            if (max(self.cube_shape) <= 1024 and self.n1 * self.n2 * self.nv < 1 << 32
+                    and self.use_det_plan):
+                c = self._call_plain_bpj()
+                p = N.TgBpjPlan()
+                p.order = 3
+                ok = N.lib().tg_bpj_plan_init(ctypes.byref(c), ctypes.byref(p)) == N.TG_OK
+                self._det_ok = ok and int(p.scratch_bytes) <= self.DET_PLAN_MAX_SCRATCH
+        return self._det_ok
+
+    DET_PLAN_MAX_SCRATCH = 8 << 30
+
+    def _call_plain_bpj(self):
+        """A backprojection call of this geometry without the deterministic flag
+        (plan set-up does not look at it)."""
+        det, self.deterministic = self.deterministic, False
+        acc, self.acc64 = self.acc64, False
+        try:
+            return self._call(N.TG_BPJ, self.xin, self.xout)
+        finally:
+            self.deterministic = det
+            self.acc64 = acc
+
+    def build_det_plan(self):
+        """Builds (once) the canonical plan of this geometry (order 3)."""
+        if self._det_plan is not None:
+            return self._det_plan
+        lib = N.lib()
+        c = self._call_plain_bpj()
+        p = N.TgBpjPlan()
+        p.order = 3
+        N.check(lib.tg_bpj_plan_init(ctypes.byref(c), ctypes.byref(p)))
+        meta = torch.empty(max(int(p.meta_bytes), 1), dtype=torch.uint8, device=self.dev)
+        p.meta = meta.data_ptr()
+        p.meta_bytes = meta.numel()
+        scratch = torch.empty(max(int(p.scratch_bytes), 1), dtype=torch.uint8, device=self.dev)
+        p.scratch = scratch.data_ptr()
+        p.scratch_bytes = scratch.numel()
+        N.check(lib.tg_bpj_plan_count(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
+        rec = torch.empty(max(int(p.rec_bytes), 1), dtype=torch.uint8, device=self.dev)
+        p.rec = rec.data_ptr()
+        p.rec_bytes = rec.numel()
+        N.check(lib.tg_bpj_plan_build(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
+        p.scratch = None
+        p.scratch_bytes = 0
+        p.pairs_ready = 0
+        del scratch
+        self._det_plan = p
+        self._det_plan_bufs = (meta, rec)
+        return p
+
     # -- calls -------------------------------------------------------------
     def _call(self, kind, cube, data, counted=True, apply_vox=True, vox_out=None):
         """The TgCall of one projection of `cube` into `data` (or back).
@@ -288,10 +351,12 @@ class Siddon:
                         tile_j=self.tile_j, pixmask=self.pixmask, pix_weight=self.pix_weight,
                         voxmask=vm, status=self.status)
         if flags & N.TG_DETERMINISTIC:
-            if self._ws is None:
-                self._ws = S.det_workspace(c, self.dev)
-            c.workspace = self._ws.data_ptr()
-            c.workspace_bytes = self._ws.numel()
+            if not self.det_plan_ok():
+                # the workspace of the unplanned deterministic mode
+                if self._ws is None:
+                    self._ws = S.det_workspace(c, self.dev)
+                c.workspace = self._ws.data_ptr()
+                c.workspace_bytes = self._ws.numel()
         elif kind in (N.TG_BPJ, N.TG_BPJT) and self.acc64 and self._dense(cube):
             c.flags |= N.TG_ACC_F64
             c.acc = self.acc_buffer().data_ptr()
@@ -418,6 +483,13 @@ class Siddon:
         return buf
 
     def _backproject_raw(self, y, out):
+        if self.deterministic and self.det_plan_ok():
+            y = self._view_major(y)
+            c = self._call(N.TG_BPJ, out, y)
+            p = self.build_det_plan()
+            N.check(N.lib().tg_bpj_plan_backproject(ctypes.byref(c), ctypes.byref(p),
+                                                    S.stream_handle()))
+            return
         if self.use_plan():
             y = self._view_major(y)
         c = self._call(N.TG_BPJ, out, y)

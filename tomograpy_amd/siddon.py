@@ -174,11 +174,15 @@ def fixed_stream():
 
 
 def det_workspace(call, dev, budget_bytes=None):
-    """Workspace tensor for deterministic backprojection of `call`."""
+    """Workspace tensor for deterministic backprojection of `call`: room for the
+    canonical brick plan the call builds in it (tg_det_plan_workspace_bytes, an
+    upper bound) when that fits in half the free HBM, else min(half the free
+    HBM, 16 GiB) for the chunked passes."""
     need = N.lib().tg_det_workspace_min_bytes(ctypes.byref(call))
     if budget_bytes is None:
         free, _ = torch.cuda.mem_get_info(dev)
-        budget_bytes = min(int(free * 0.5), 16 << 30)
+        plan = int(N.lib().tg_det_plan_workspace_bytes(ctypes.byref(call)))
+        budget_bytes = plan if 0 < plan <= free // 2 else min(int(free * 0.5), 16 << 30)
     nbytes = max(int(need), int(budget_bytes))
     return torch.empty(nbytes, dtype=torch.uint8, device=dev)
 

@@ -121,3 +121,21 @@ def test_c2_deterministic_16_views_bit_exact(oracle):
     oracle.siddon(oracle.BPJ, view_records(hs), map_record(ch), ref, y, nthread=1)
     got = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_deterministic_long_buckets(oracle, dtype):
+    """Many rays per voxel (a 12^3 cube under 160^2 pixels, about 13 pixels
+    per voxel edge): a batch gives a voxel far more than 8 contributions, so
+    the in-place sort of long buckets runs beside the register sort of short
+    ones.  Bit-exact to the oracle's single-thread order."""
+    print("deterministic long buckets", np.dtype(dtype).name, flush=True)
+    ch, x, data = C.small_case(n=12, npix=160, nv=5, dtype=dtype, seed=3)
+    P = Siddon(data.header, ch, dtype=dtype, deterministic=True)
+    assert P.det_plan_ok()
+    rng = np.random.default_rng(20)
+    y = rng.uniform(-1, 1, data.shape).astype(dtype)
+    ref = np.zeros(x.shape, dtype)
+    oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref, y, nthread=1)
+    got = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
+    assert np.array_equal(got, ref)

@@ -253,6 +253,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
 // run adds into the brick's voxels (C:489-494 with nthread = 1).
 constexpr int kDetMaxBricks = 2048;  // 256-bit lane masks of all bricks in LDS (64 KiB)
 constexpr int kDetSegRows = 256;     // rows per segment of the column scan
+constexpr int kDetMinSlots = 256;    // fewest contribution slots of one deterministic batch
 
 // Segment sums of the columns: block (s, cb) sums rows [s * 256, (s + 1) * 256)
 // of columns [cb * 256, cb * 256 + 256) into seg[s * nb + col].
@@ -611,6 +612,21 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *wsum) 
   return v + wsum[wave];
 }
 
+// Ascending order of 8 values: Batcher's odd-even merge network (19 exchanges).
+__device__ __forceinline__ void sort8(uint32_t (&v)[8]) {
+  auto cx = [&](int i, int j) {
+    const uint32_t lo = v[i] < v[j] ? v[i] : v[j], hi = v[i] < v[j] ? v[j] : v[i];
+    v[i] = lo;
+    v[j] = hi;
+  };
+  cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+  cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+  cx(1, 2); cx(5, 6);
+  cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+  cx(2, 4); cx(3, 5);
+  cx(1, 2); cx(3, 4); cx(5, 6);
+}
+
 template <typename T>
 __host__ __device__ __forceinline__ size_t det_lds_bytes(int cells, int mcap) {
   return (size_t)cells * sizeof(T) + (size_t)mcap * sizeof(T) + (size_t)cells * 4 +
@@ -645,39 +661,48 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
     const int64_t i0 = lo0 + l0, i1 = lo1 + l1, i2 = lo2 + l2;
     const bool inside = i0 < (int64_t)a.n[0] && i1 < (int64_t)a.n[1] && i2 < (int64_t)a.n[2];
     acc[c] = inside ? a.cube[boff + i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2]] : (T)0;
+    cnt[c] = 0u;
   }
   const bool view_major = a.ds[1] == 1 && a.ds[0] == (int64_t)a.n2 &&
                           a.ds[2] == (int64_t)a.n1 * (int64_t)a.n2;
   const int dm0 = brick_pitch0(g), dm1 = brick_pitch1(g);
   unsigned long long r = offsets[b];
   const unsigned long long r1 = offsets[b + 1];
+  // a lane's record: streamed (ray id, lq, entry distances), then gathered by
+  // ray id (pixel value, ray entry).  The next batch's records are loaded
+  // while this batch is summed.
+  auto stream_rec = [&](unsigned long long p, uint32_t &rid, uint32_t &lq, T &d0, T &d1, T &d2) {
+    rid = rec.rid[p];
+    lq = rec.lq[p];
+    d0 = rec.d0[p];
+    d1 = rec.d1[p];
+    d2 = rec.d2[p];
+  };
+  auto gather_rec = [&](uint32_t rid, T &w, RayEntry<T> &e) {
+    int64_t did = rid;
+    if (!view_major) {
+      int t, i, j;
+      ray_pixel((uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
+      did = i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+    }
+    w = a.data[did];
+    if (a.pixw) w = a.pixw[did] * w;
+    e = rec.ray[rid];
+  };
+  uint32_t rid = 0, lq = 0;
+  T D0 = 0, D1 = 0, D2 = 0, w = 0;
+  RayEntry<T> e;
+  e.pabs[0] = e.pabs[1] = e.pabs[2] = (T)0;
+  e.up = 0;
+  if (r + tid < r1) {
+    stream_rec(r + tid, rid, lq, D0, D1, D2);
+    gather_rec(rid, w, e);
+  }
   __syncthreads();
   while (r < r1) {
-    const unsigned long long p = r + tid;
-    const bool valid = p < r1;
-    uint32_t lq = 0, n = 0;
-    T D0 = 0, D1 = 0, D2 = 0, w = 0;
-    RayEntry<T> e;
-    e.pabs[0] = e.pabs[1] = e.pabs[2] = (T)0;
-    e.up = 0;
-    if (valid) {
-      const uint32_t rid = rec.rid[p];
-      lq = rec.lq[p];
-      D0 = rec.d0[p];
-      D1 = rec.d1[p];
-      D2 = rec.d2[p];
-      int64_t did = rid;
-      if (!view_major) {
-        int t, i, j;
-        ray_pixel((uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
-        did = i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
-      }
-      w = a.data[did];
-      if (a.pixw) w = a.pixw[did] * w;
-      e = rec.ray[rid];
-      // a zero pixel adds zeros: its steps are left out
-      n = w != (T)0 ? lq >> 16 : 0u;
-    }
+    const bool valid = r + tid < r1;
+    // a zero pixel adds zeros: its steps are left out
+    const uint32_t n = valid && w != (T)0 ? lq >> 16 : 0u;
     if (tid == 0) {
       batch[0] = 0;
       batch[1] = 0;
@@ -688,9 +713,13 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
       atomicAdd(&batch[0], 1u);
       atomicMax(&batch[1], incl);
     }
-    for (int c = tid; c < cells; c += kBpjBlock) cnt[c] = 0;
-    __syncthreads();
+    __syncthreads();  // the buckets cnt[] start at zero (left so by the last batch)
     const uint32_t nrec = batch[0], nslot = batch[1];
+    const unsigned long long pn = r + nrec + tid;
+    const bool vn = pn < r1;
+    uint32_t rid_n = 0, lq_n = 0;
+    T E0 = 0, E1 = 0, E2 = 0;
+    if (vn) stream_rec(pn, rid_n, lq_n, E0, E1, E2);
     if (take && n) {
       const int u0 = ((e.up >> 0) & 3) - 1, u1 = ((e.up >> 2) & 3) - 1, u2 = ((e.up >> 4) & 3) - 1;
       int cell = (int)(lq & 0xffffu);
@@ -708,46 +737,98 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
       }
     }
     __syncthreads();
-    // bucket starts: cnt[c] = (exclusive prefix << 16) | count
+    // bucket starts: cnt[c] = (exclusive prefix << 16) | count; the cells with
+    // contributions are counted in the low half of the same scan
     const int per = (cells + kBpjBlock - 1) / kBpjBlock;
     const int c0 = (int)tid * per;
     uint32_t local = 0;
-    for (int c = c0; c < c0 + per && c < cells; c++) local += cnt[c];
-    uint32_t run = block_incl_scan(local, wsum) - local;
     for (int c = c0; c < c0 + per && c < cells; c++) {
       const uint32_t k = cnt[c];
-      cnt[c] = (run << 16) | k;
+      local += (k << 16) | (k != 0u ? 1u : 0u);
+    }
+    const uint32_t excl = block_incl_scan(local, wsum) - local;
+    const uint32_t nact = wsum[kBpjBlock / 64] & 0xffffu;
+    uint32_t run = excl >> 16;
+    for (int c = c0; c < c0 + per && c < cells; c++) {
+      const uint32_t k = cnt[c];
+      cnt[c] = k ? (run << 16) | k : 0u;
       run += k;
     }
     __syncthreads();
-    for (uint32_t m = tid; m < nslot; m += kBpjBlock) {
-      const uint32_t c = ucell[m];
-      const uint32_t pos = atomicAdd(&cnt[c], 1u << 16) >> 16;
-      sidx[pos] = (uint16_t)m;
+    // four slots per lane at a time: their reads and atomics overlap
+    for (uint32_t m0 = tid; m0 < nslot; m0 += 4 * kBpjBlock) {
+      uint32_t cs[4], pos[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t m = m0 + j * kBpjBlock;
+        cs[j] = m < nslot ? (uint32_t)ucell[m] : 0xffffffffu;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (cs[j] != 0xffffffffu) pos[j] = atomicAdd(&cnt[cs[j]], 1u << 16) >> 16;
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (cs[j] != 0xffffffffu) sidx[pos[j]] = (uint16_t)(m0 + j * kBpjBlock);
     }
     __syncthreads();
-    // each cell adds its bucket in slot order, one contribution at a time
-    for (int c = tid; c < cells; c += kBpjBlock) {
+    T w_n = 0;
+    RayEntry<T> e_n;
+    e_n.pabs[0] = e_n.pabs[1] = e_n.pabs[2] = (T)0;
+    e_n.up = 0;
+    if (vn) gather_rec(rid_n, w_n, e_n);
+    // the cells with contributions, listed over the slots' cells (nact <= nslot)
+    uint32_t ap = excl & 0xffffu;
+    for (int c = c0; c < c0 + per && c < cells; c++)
+      if (cnt[c] & 0xffffu) ucell[ap++] = (uint16_t)c;
+    __syncthreads();
+    // each listed cell adds its bucket in slot order, one contribution at a
+    // time, and leaves its bucket word zero for the next batch.  The slot
+    // numbers of a bucket of up to 8 are sorted in registers (a sorting
+    // network), longer buckets in place.
+    for (uint32_t q = tid; q < nact; q += kBpjBlock) {
+      const int c = ucell[q];
       const uint32_t v = cnt[c];
+      cnt[c] = 0u;
       const uint32_t k = v & 0xffffu;
-      if (!k) continue;
       const uint32_t end = v >> 16, st = end - k;
-      for (uint32_t q = st + 1; q < end; q++) {  // insertion sort: buckets are short
-        const uint16_t x = sidx[q];
-        uint32_t z = q;
-        while (z > st && sidx[z - 1] > x) {
-          sidx[z] = sidx[z - 1];
-          z--;
-        }
-        sidx[z] = x;
-      }
       T sum = acc[c];
-      for (uint32_t q = st; q < end; q++) sum = sum + uval[sidx[q]];
+      if (k <= 8u) {
+        uint32_t sl[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) sl[i] = (uint32_t)i < k ? (uint32_t)sidx[st + i] : 0xffffu;
+        sort8(sl);
+        T u[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) u[i] = (uint32_t)i < k ? uval[sl[i]] : (T)0;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+          if ((uint32_t)i < k) sum = sum + u[i];
+      } else {
+        for (uint32_t i = st + 1; i < end; i++) {  // insertion sort
+          const uint16_t x = sidx[i];
+          uint32_t z = i;
+          while (z > st && sidx[z - 1] > x) {
+            sidx[z] = sidx[z - 1];
+            z--;
+          }
+          sidx[z] = x;
+        }
+        for (uint32_t i = st; i < end; i++) sum = sum + uval[sidx[i]];
+      }
       acc[c] = sum;
     }
-    __syncthreads();
     r += nrec;
+    rid = rid_n;
+    lq = lq_n;
+    D0 = E0;
+    D1 = E1;
+    D2 = E2;
+    w = w_n;
+    e = e_n;
   }
+  // the last batch's sums are complete (inside the loop the next batch's
+  // scans order them before the slots are rewritten)
+  __syncthreads();
   for (int c = tid; c < cells; c += kBpjBlock) {
     const int l2 = c & (g.b[2] - 1), l1 = (c >> g.lg[2]) & (g.b[1] - 1), l0 = c >> (g.lg[2] + g.lg[1]);
     const int64_t i0 = lo0 + l0, i1 = lo1 + l1, i2 = lo2 + l2;
@@ -843,6 +924,10 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
       return set_error(TG_EINVAL, "a canonical plan traces rows of %d pixels", kPlanTraceTile);
     if (g.b[0] * g.b[1] * g.b[2] > 8192)
       return set_error(TG_EINVAL, "a canonical plan's bricks hold at most 8192 voxels");
+    // one record's steps (at most the three edges' sum) must fit the smallest
+    // batch of the deterministic kernel, or a batch could take no record
+    if (g.b[0] + g.b[1] + g.b[2] > kDetMinSlots)
+      return set_error(TG_EINVAL, "a canonical plan's brick edges sum to at most %d", kDetMinSlots);
   }
   return TG_OK;
 }
@@ -1139,7 +1224,7 @@ int launch_plan_bpj(const KArgs<T> &a, const BrickGeom &g, const tg_bpj_plan *p,
 template <typename T>
 int det_slots(int cells) {
   int mcap = 8192;
-  while (mcap > 256 && det_lds_bytes<T>(cells, mcap) > (size_t)kPlanLdsBytes) mcap >>= 1;
+  while (mcap > kDetMinSlots && det_lds_bytes<T>(cells, mcap) > (size_t)kPlanLdsBytes) mcap >>= 1;
   return mcap;
 }
 

@@ -86,8 +86,16 @@ __device__ __forceinline__ void ray_pixel(uint32_t n1, uint32_t n2, uint32_t rid
 template <typename T>
 struct RayEntry {
   T pabs[3];
-  int32_t up;
+  int32_t up;  // step direction (-1, 0, 1) per axis: 2-bit two's complement at bits 0, 2, 4
 };
+
+__host__ __device__ __forceinline__ int32_t pack_up(const int up[3]) {
+  return (up[0] & 3) | ((up[1] & 3) << 2) | ((up[2] & 3) << 4);
+}
+// Axis k's direction: one signed bitfield extract.
+__host__ __device__ __forceinline__ int up_axis(int32_t up, int k) {
+  return (int32_t)((uint32_t)up << (30 - 2 * k)) >> 30;
+}
 
 // Records, structure of arrays: ray id, lq = (voxel index inside the brick) |
 // (steps in the brick) << 16, and the distances D at the brick entry.
@@ -325,7 +333,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, Bric
       e.pabs[0] = r.pabs[0];
       e.pabs[1] = r.pabs[1];
       e.pabs[2] = r.pabs[2];
-      e.up = (r.up[0] + 1) | ((r.up[1] + 1) << 2) | ((r.up[2] + 1) << 4);
+      e.up = pack_up(r.up);
       rec.ray[rid] = e;
     }
     const uint32_t *m = &mask[b * 8];
@@ -382,7 +390,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
       e.pabs[0] = r.pabs[0];
       e.pabs[1] = r.pabs[1];
       e.pabs[2] = r.pabs[2];
-      e.up = (r.up[0] + 1) | ((r.up[1] + 1) << 2) | ((r.up[2] + 1) << 4);
+      e.up = pack_up(r.up);
       rec.ray[rid] = e;
     }
     const uint32_t s = lds ? atomicAdd(&slot[b], 1u) : atomicAdd(&cursor[b], 1u);
@@ -491,6 +499,20 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_sort(BrickGeom g,
   }
 }
 
+// base[i] for a wavefront-uniform global base and a lane's 32-bit index: the
+// base is moved to scalar registers so that the load takes the scalar base +
+// 32-bit offset form, with no per-lane 64-bit address arithmetic.
+template <typename V>
+__device__ __forceinline__ V lane_load(const V *base, uint32_t i) {
+  typedef const __attribute__((address_space(1))) V *global_ptr;
+  const uint64_t v = (uint64_t)base;
+  // (the builtin returns int: each half is taken as unsigned before widening)
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint64_t u = ((uint64_t)hi << 32) | (uint64_t)lo;
+  return ((global_ptr)u)[i];
+}
+
 // The work of one workgroup: chunk c of one brick's records, accumulated in
 // the LDS array acc and flushed to the cube (or to the float64 accumulator).
 template <typename T>
@@ -518,30 +540,40 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
   const bool view_major = a.ds[1] == 1 && a.ds[0] == (int64_t)a.n2 &&
                           a.ds[2] == (int64_t)a.n1 * (int64_t)a.n2;
   for (unsigned long long pw = r0 + wave0; pw < r1; pw += blockDim.x) {
-    const bool valid = pw + lane < r1;
-    const unsigned long long p = valid ? pw + lane : r1 - 1;
-    // the record's own fields in one round trip, then the two gathers that
-    // depend on its ray id (pixel value, ray constants) in a second one
-    const uint32_t rid = rec.rid[p];
-    const uint32_t lq = rec.lq[p];
-    T D0 = rec.d0[p], D1 = rec.d1[p], D2 = rec.d2[p];
-    int64_t did = rid;
-    if (!view_major) {
-      int t, i, j;
-      ray_pixel((uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
-      did = a.data32 ? (int64_t)((uint32_t)i * (uint32_t)a.ds[0] + (uint32_t)j * (uint32_t)a.ds[1] +
-                                 (uint32_t)t * (uint32_t)a.ds[2])
-                     : i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+    const unsigned long long left = r1 - pw;  // records from this group's first
+    const bool valid = left >= 64u || lane < (uint32_t)left;
+    // the record's own fields in one round trip (a scalar base for the
+    // wavefront's 64 records, the lane's constant byte offset), then the two
+    // gathers that depend on its ray id (pixel value, ray constants) in a
+    // second one.  Lanes past the chunk's end load nothing and walk no steps.
+    uint32_t lq = 0, rid;
+    T D0, D1, D2, w;
+    RayEntry<T> e;
+    if (valid) {
+      rid = lane_load(rec.rid + pw, lane);
+      lq = lane_load(rec.lq + pw, lane);
+      D0 = lane_load(rec.d0 + pw, lane);
+      D1 = lane_load(rec.d1 + pw, lane);
+      D2 = lane_load(rec.d2 + pw, lane);
+      int64_t did = rid;
+      if (!view_major) {
+        int t, i, j;
+        ray_pixel((uint32_t)a.n1, (uint32_t)a.n2, rid, t, i, j);
+        did = a.data32 ? (int64_t)((uint32_t)i * (uint32_t)a.ds[0] + (uint32_t)j * (uint32_t)a.ds[1] +
+                                   (uint32_t)t * (uint32_t)a.ds[2])
+                       : i * a.ds[0] + j * a.ds[1] + (int64_t)t * a.ds[2];
+      }
+      w = a.data[did];
+      e = rec.ray[rid];
+      if (a.pixw) w = a.pixw[did] * w;
     }
-    T w = a.data[did];
-    const RayEntry<T> e = rec.ray[rid];
-    if (a.pixw) w = a.pixw[did] * w;
     // a zero pixel value walks its steps adding zeros, which keeps the
     // wavefront together and leaves every sum unchanged
     // byte offsets in the double accumulators of one step along each axis
-    const int u0 = ((e.up >> 0) & 3) - 1, u1 = ((e.up >> 2) & 3) - 1, u2 = ((e.up >> 4) & 3) - 1;
-    const int dm0 = u0 * brick_pitch0(g) * (int)sizeof(Acc);
-    const int dm1 = u1 * brick_pitch1(g) * (int)sizeof(Acc);
+    const int u0 = up_axis(e.up, 0), u1 = up_axis(e.up, 1), u2 = up_axis(e.up, 2);
+    // (24-bit products: |pitch| x 8 < 2^23 for bricks of at most 65536 cells)
+    const int dm0 = __mul24(u0, brick_pitch0(g) * (int)sizeof(Acc));
+    const int dm1 = __mul24(u1, brick_pitch1(g) * (int)sizeof(Acc));
     const int dm2 = u2 * (int)sizeof(Acc);
     unsigned char *cell = (unsigned char *)(acc + (lq & 0xffffu));
     const uint32_t n = valid ? lq >> 16 : 0u;
@@ -721,7 +753,7 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
     T E0 = 0, E1 = 0, E2 = 0;
     if (vn) stream_rec(pn, rid_n, lq_n, E0, E1, E2);
     if (take && n) {
-      const int u0 = ((e.up >> 0) & 3) - 1, u1 = ((e.up >> 2) & 3) - 1, u2 = ((e.up >> 4) & 3) - 1;
+      const int u0 = up_axis(e.up, 0), u1 = up_axis(e.up, 1), u2 = up_axis(e.up, 2);
       int cell = (int)(lq & 0xffffu);
       uint32_t m = incl - n;
       for (uint32_t k = 0; k < n; k++, m++) {

@@ -263,7 +263,9 @@ def batch_leg():
         P.ray_steps()
         if P.use_plan():
             P.build_plan()
-    tuner = B.Autotuner(repeats=1)
+    # each (tile width, schedule) point timed 3 times after a warm-up; the
+    # fastest of its 3 decides
+    tuner = B.Autotuner(repeats=3)
 
     def batched():
         B.run_tuned(tuner, ops, xs, outs)
@@ -274,8 +276,16 @@ def batch_leg():
             P.project_into(x, o)
             P.backproject_into(o, c)
 
+    batched()  # tunes the forward launch
+    torch.cuda.synchronize()
+    trials = list(tuner.trials.values())[0]
+    # marker ranges (roctx) that a trace of this run can cut its phases by
+    torch.cuda.nvtx.range_push("batch batched")
     ms_b = _event_ms(batched, 3)
+    torch.cuda.nvtx.range_pop()
+    torch.cuda.nvtx.range_push("batch one-by-one")
     ms_1 = _event_ms(one_by_one, 3)
+    torch.cuda.nvtx.range_pop()
     steps = sum(P.count_steps() for P in ops)
     # one batched pair from zero against the same jobs run one by one in float64
     for o, c in zip(outs, cubes):
@@ -292,6 +302,9 @@ def batch_leg():
     return {"workload": "64 jobs, cubes 32^3-192^3, 8-96 views of 64^2-512^2, half Thomson, fp32",
             "jobs": len(ops), "ms_batched_pair": ms_b, "ms_one_by_one_pair": ms_1,
             "speedup": ms_1 / ms_b, "siddon_steps_per_pair": 2 * steps,
+            "forward_plan": {"tile_j": trials[0][1], "schedule": trials[0][2],
+                             "repeats": tuner.repeats,
+                             "trials_ms": [[round(t, 4), tj, sch] for t, tj, sch in trials]},
             "steps_per_s_batched": 2 * steps / (ms_b * 1e-3),
             "error_vs_fp64": {"forward_rel_l2_max": max(err_f),
                               "forward_rel_l2_median": float(np.median(err_f)),

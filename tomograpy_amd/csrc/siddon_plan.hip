@@ -445,50 +445,129 @@ __device__ __forceinline__ uint32_t tie_pos(uint32_t x, int s) {
   return (x & ~m) | (__brev(x & m) >> (32 - s));
 }
 
+// Inclusive scan of v over the workgroup (kBpjBlock lanes); wsum holds
+// kBpjBlock / 64 + 1 words of LDS.
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *wsum) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o, 64);
+    if (lane >= (uint32_t)o) v += u;
+  }
+  if (lane == 63u) wsum[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t w = 0; w < kBpjBlock / 64; w++) {
+      const uint32_t x = wsum[w];
+      wsum[w] = run;
+      run += x;
+    }
+    wsum[kBpjBlock / 64] = run;
+  }
+  __syncthreads();
+  return v + wsum[wave];
+}
+
+// Sort of the records of each pass (kBpjBlock consecutive records of a
+// chunk) by decreasing step count, ties by tie_pos of the pass position:
+// the ascending order of the keys ((0xffff - steps) << 10) | tie_pos(position).
+// A pass whose records all take fewer than 256 steps (every pass of a usual
+// brick) is sorted by one stable counting pass over the digit 255 - steps in
+// tie_pos order; any other pass by a bitonic network over the full keys.  The
+// two give the same order.
+constexpr int kSortWaves = kBpjBlock / 64;
+constexpr int kSortBuckets = 257;  // digits 0..255, and 256 for lanes past the chunk
+
+// (at most 64 VGPRs: two workgroups per CU)
 template <typename T>
-__global__ __launch_bounds__(kBpjBlock) void k_plan_sort(BrickGeom g,
+__global__ __launch_bounds__(kBpjBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_plan_sort(
+    BrickGeom g,
                                                          const unsigned long long *offsets,
                                                          const unsigned long long *chunk_prefix,
                                                          RecPtrs<T> rec) {
-  __shared__ uint32_t key[kBpjBlock];
+  __shared__ uint32_t key[kBpjBlock];                         // keys, or each pass's source lanes
+  __shared__ uint32_t hist[kSortBuckets * kSortWaves];        // (digit, wavefront) counts
+  __shared__ uint32_t wsum[kSortWaves + 1];
+  __shared__ uint16_t dig[kBpjBlock];
   __shared__ uint32_t s_rid[kBpjBlock], s_lq[kBpjBlock];
   __shared__ T s_d0[kBpjBlock], s_d1[kBpjBlock], s_d2[kBpjBlock];
   unsigned long long r0, r1;
   chunk_range(g, offsets, chunk_prefix, blockIdx.x, r0, r1);
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (unsigned long long ws = r0; ws < r1; ws += kBpjBlock) {
     const unsigned long long q = ws + tid;
-    if (q < r1) {
+    const bool valid = q < r1;
+    uint32_t steps = 0;
+    if (valid) {
       s_rid[tid] = rec.rid[q];
       s_lq[tid] = rec.lq[q];
       s_d0[tid] = rec.d0[q];
       s_d1[tid] = rec.d1[q];
       s_d2[tid] = rec.d2[q];
-      // longest first; ties by pass position with its low g.spread bits
-      // reversed: spread, the walks a wavefront takes together come from rays
-      // apart in the pass rather than from adjacent pixels, which in a coarse
-      // cube step through the same voxels and serialise on the same LDS cells
-      key[tid] = ((0xffffu - (s_lq[tid] >> 16)) << 10) | tie_pos(tid, g.spread);
+      steps = s_lq[tid] >> 16;
+    }
+    // longest first; ties by pass position with its low g.spread bits
+    // reversed: spread, the walks a wavefront takes together come from rays
+    // apart in the pass rather than from adjacent pixels, which in a coarse
+    // cube step through the same voxels and serialise on the same LDS cells
+    const uint32_t tp = tie_pos(tid, g.spread);
+    if (__syncthreads_or(valid && steps > 255u)) {
+      key[tid] = valid ? ((0xffffu - steps) << 10) | tp : 0xffffffffu;
+      __syncthreads();
+      for (uint32_t k = 2; k <= kBpjBlock; k <<= 1) {  // bitonic sort, ascending
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+          const uint32_t o = tid ^ j;
+          if (o > tid) {
+            const uint32_t x = key[tid], y = key[o];
+            const bool up = (tid & k) == 0;
+            if ((x > y) == up) {
+              key[tid] = y;
+              key[o] = x;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      // tie_pos is an involution: the key's position part names the source lane
+      key[tid] = tie_pos(key[tid] & (kBpjBlock - 1), g.spread);
     } else {
-      key[tid] = 0xffffffffu;
+      // lane tid's digit at its place tp in tie order
+      dig[tp] = (uint16_t)(valid ? 255u - steps : 256u);
+      for (int e = tid; e < kSortBuckets * kSortWaves; e += kBpjBlock) hist[e] = 0;
+      __syncthreads();
+      // element v = tid of the tie order: its rank among the equal digits of
+      // its wavefront (lanes agree on all 9 digit bits), and the wavefront's
+      // count of that digit
+      const uint32_t d = dig[tid];
+      unsigned long long peers = ~0ull;
+#pragma unroll
+      for (int bit = 0; bit < 9; bit++) {
+        const unsigned long long b = __ballot((d >> bit) & 1u);
+        peers &= ((d >> bit) & 1u) ? b : ~b;
+      }
+      const uint32_t rank = (uint32_t)__popcll(peers & below);
+      if (rank == 0) hist[d * kSortWaves + wave] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      // exclusive scan of the counts in (digit, wavefront) order
+      constexpr int per = (kSortBuckets * kSortWaves + kBpjBlock - 1) / kBpjBlock;
+      const int e0 = (int)tid * per;
+      uint32_t local = 0;
+      for (int e = e0; e < e0 + per && e < kSortBuckets * kSortWaves; e++) local += hist[e];
+      uint32_t run = block_incl_scan(local, wsum) - local;
+      for (int e = e0; e < e0 + per && e < kSortBuckets * kSortWaves; e++) {
+        const uint32_t c = hist[e];
+        hist[e] = run;
+        run += c;
+      }
+      __syncthreads();
+      // sorted position of element v; its record is lane tie_pos(v)
+      key[hist[d * kSortWaves + wave] + rank] = tie_pos(tid, g.spread);
     }
     __syncthreads();
-    for (uint32_t k = 2; k <= kBpjBlock; k <<= 1) {  // bitonic sort, ascending
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        const uint32_t o = tid ^ j;
-        if (o > tid) {
-          const uint32_t x = key[tid], y = key[o];
-          const bool up = (tid & k) == 0;
-          if ((x > y) == up) {
-            key[tid] = y;
-            key[o] = x;
-          }
-        }
-        __syncthreads();
-      }
-    }
-    if (q < r1) {
-      const uint32_t src = tie_pos(key[tid] & (kBpjBlock - 1), g.spread);
+    if (valid) {
+      const uint32_t src = key[tid];
       rec.rid[q] = s_rid[src];
       rec.lq[q] = s_lq[src];
       rec.d0[q] = s_d0[src];
@@ -620,29 +699,6 @@ __device__ __forceinline__ void plan_bpj_chunk(const KArgs<T> &a, const BrickGeo
 // (view, row, column, C:489-494), starting from the cube's value, and the
 // result is bit-identical to that run.
 
-// Inclusive scan of v over the workgroup (kBpjBlock lanes); wsum holds
-// kBpjBlock / 64 + 1 words of LDS.
-__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *wsum) {
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t u = __shfl_up(v, o, 64);
-    if (lane >= (uint32_t)o) v += u;
-  }
-  if (lane == 63u) wsum[wave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (uint32_t w = 0; w < kBpjBlock / 64; w++) {
-      const uint32_t x = wsum[w];
-      wsum[w] = run;
-      run += x;
-    }
-    wsum[kBpjBlock / 64] = run;
-  }
-  __syncthreads();
-  return v + wsum[wave];
-}
 
 // Ascending order of 8 values: Batcher's odd-even merge network (19 exchanges).
 __device__ __forceinline__ void sort8(uint32_t (&v)[8]) {

@@ -597,7 +597,41 @@ def cpu_baseline(ch, hs, cube_np, data_np, target_s):
     dt1 = time.perf_counter() - t0
     cpu["single_thread"] = {"value": (s1 + s2) / dt1, "cores": 1,
                             "sample": "oracle pj+bpj on views 0..%d, %.1f s" % (k1 - 1, dt1)}
+    cpu["config1"] = cpu_config1(threads)
     return cpu, d[..., :k2], c, k2
+
+
+def cpu_config1(threads):
+    """The oracle on the whole C1 workload (64^3, 32 views of 128^2): steps/s of
+    the forward and of the backward alone, fp32 and fp64, on all host threads
+    and on one; median of 5 runs each."""
+    from oracle import oracle as O
+    from tomograpy_amd import workloads as W
+    from tomograpy_amd.headers import map_record, view_records
+    out = {"workload": "C1: 64^3 cube, 32 views of 128^2", "runs": 5}
+    rng = np.random.default_rng(W.SEED0 + 9)
+    for dtype in (np.float32, np.float64):
+        ch, cube, hs = W.config1(dtype)
+        recs, mrec = view_records(hs), map_record(ch)
+        x = np.ascontiguousarray(cube, dtype=dtype)
+        y = rng.uniform(0, 1, (128, 128, len(hs))).astype(dtype)
+        res = {}
+        for nt in (threads, 1):
+            for name, kind in (("forward", O.PJ), ("backward", O.BPJ)):
+                rates = []
+                for _ in range(5):
+                    d = np.zeros_like(y)
+                    c = np.zeros_like(x)
+                    t0 = time.perf_counter()
+                    steps = (O.siddon(kind, recs, mrec, x, d, nthread=nt) if kind == O.PJ
+                             else O.siddon(kind, recs, mrec, c, y, nthread=nt))
+                    rates.append(steps / (time.perf_counter() - t0))
+                res["%s_%s" % (name, "all_threads" if nt == threads else "one_thread")] = \
+                    float(np.median(rates))
+        out[np.dtype(dtype).name] = res
+    out["threads"] = threads
+    out["unit"] = "Siddon steps/s"
+    return out
 
 
 def sample_parity(ch, hs, dtype, cube_np, data_np, d_ref, c_ref, k):

@@ -299,7 +299,8 @@ int tg_smoothness(int32_t dtype, const int32_t shape[4], const double weights[4]
 /* ---- segmented CG: the vector work of many concurrent solves in one launch
  * (no reference counterpart; used by the reconstruction server) ---- */
 
-/* One solve's vectors (contiguous, n elements each) and its regulariser. */
+/* One solve's vectors (contiguous, n elements each, four distinct arrays that do not overlap)
+ * and its regulariser. */
 typedef struct tg_cg_seg {
   void *x, *r, *p, *q;
   const uint8_t *voxmask; /* may be NULL */
@@ -320,6 +321,7 @@ typedef struct tg_cg_seg {
 #define TG_CG_OP_DIRECTION 3  /* beta = scal[2]/scal[0]; p = r + beta p; scal[0] = scal[2] */
 #define TG_CG_OP_INIT 4       /* scal[0] = scal[5] = r.r */
 #define TG_CG_OP_ZERO_Q 5     /* q = 0 */
+#define TG_CG_OP_REGULARIZE_DOT 6 /* REGULARIZE, then DOT_PQ, in one pass over the elements */
 
 /* Launch blocks a segment of n elements takes. */
 int64_t tg_cg_seg_blocks(int64_t n);

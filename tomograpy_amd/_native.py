@@ -55,7 +55,7 @@ class TgCgSeg(ctypes.Structure):
 
 
 (TG_CG_OP_REGULARIZE, TG_CG_OP_DOT_PQ, TG_CG_OP_UPDATE, TG_CG_OP_DIRECTION, TG_CG_OP_INIT,
- TG_CG_OP_ZERO_Q) = range(6)
+ TG_CG_OP_ZERO_Q, TG_CG_OP_REGULARIZE_DOT) = range(7)
 
 
 class TgBpjPlan(ctypes.Structure):

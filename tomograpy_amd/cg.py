@@ -184,8 +184,7 @@ def iterate(solves, table, tuner=None):
         # geometries with a brick plan backproject in one plan launch, the
         # others in one persistent atomic launch
         bwd.run(plan_b)
-    table.op(N.TG_CG_OP_REGULARIZE)
-    table.op(N.TG_CG_OP_DOT_PQ)
+    table.op(N.TG_CG_OP_REGULARIZE_DOT)
     table.op(N.TG_CG_OP_UPDATE)
     table.op(N.TG_CG_OP_DIRECTION)
     for s in solves:

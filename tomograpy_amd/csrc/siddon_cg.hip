@@ -21,14 +21,51 @@ namespace {
 constexpr int kRedBlocks = 1024;  // max partials, lives at scal[16 .. 16 + kRedBlocks)
 constexpr int kScalPartials = 16;
 
+// k / d for k < 2^32 from the divisor's double reciprocal rd = 1 / d: k * rd
+// is within one of the quotient, and one test of the remainder corrects it.
+__device__ __forceinline__ uint32_t udiv_rcp(uint32_t k, uint32_t d, double rd) {
+  const uint32_t q = (uint32_t)((double)k * rd);
+  const int64_t rem = (int64_t)k - (int64_t)q * (int64_t)d;
+  return rem < 0 ? q - 1u : (rem >= (int64_t)d ? q + 1u : q);
+}
+
+// Coordinates m of flat index k in a C-order array of shape n, strides st
+// (st[3] = 1).  Arrays of at most 2^32 elements divide in 32 bits through
+// reciprocals (the integer divisions, 64-bit ones above all, are emulated at
+// many times the cost).
+__device__ __forceinline__ void grid_pos(const int64_t n[4], const int64_t st[4], int64_t k,
+                                         int64_t m[4]) {
+  if (n[0] * st[0] <= (int64_t)0xffffffff) {
+    const uint32_t kk = (uint32_t)k, s0 = (uint32_t)st[0], s1 = (uint32_t)st[1], s2 = (uint32_t)st[2];
+    const uint32_t a = udiv_rcp(kk, s0, 1.0 / (double)s0), r0 = kk - a * s0;
+    const uint32_t b = udiv_rcp(r0, s1, 1.0 / (double)s1), r1 = r0 - b * s1;
+    const uint32_t c = udiv_rcp(r1, s2, 1.0 / (double)s2);
+    m[0] = a;
+    m[1] = b;
+    m[2] = c;
+    m[3] = r1 - c * s2;
+  } else {
+    m[0] = k / st[0];
+    m[1] = (k / st[1]) % n[1];
+    m[2] = (k / st[2]) % n[2];
+    m[3] = k % n[3];
+  }
+}
+
+// Global-memory pointers: loads through them are global, not flat, loads
+// (pointers read from a table are generic otherwise).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+
 // (identity ? x_k : 0) + sum_k D_k^T D_k x at flat index k of an array of
 // shape n[4] (C order), or with the voxel weight w (not null)
-// (identity ? x_k : 0) + w_k sum_k D_k^T D_k (w x).
-template <typename T>
-__device__ __forceinline__ T reg_at(const int64_t n[4], uint32_t axes, int identity,
-                                    const T *__restrict__ x, const T *__restrict__ w, int64_t k) {
+// (identity ? x_k : 0) + w_k sum_k D_k^T D_k (w x).  P: the pointer type.
+template <typename T, typename P>
+__device__ __forceinline__ T reg_at(const int64_t n[4], uint32_t axes, int identity, P x, P w,
+                                    int64_t k) {
   const int64_t st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
-  const int64_t m[4] = {k / st[0], (k / st[1]) % n[1], (k / st[2]) % n[2], k % n[3]};
+  int64_t m[4];
+  grid_pos(n, st, k, m);
   const T xc = x[k];
   T acc = identity ? xc : (T)0;
   if (!w) {
@@ -85,9 +122,10 @@ __global__ __launch_bounds__(256) void k_smooth(int64_t n0, int64_t n1, int64_t 
   double v = 0.0;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
        k += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t m[4] = {k / s0, (k / s1) % n1, (k / s2) % n2, k % n3};
     const int64_t n[4] = {n0, n1, n2, n3};
     const int64_t st[4] = {s0, s1, s2, s3};
+    int64_t m[4];
+    grid_pos(n, st, k, m);
     const T xc = vw ? vw[k] * x[k] : x[k];
     T g = (T)0;
 #pragma unroll
@@ -206,9 +244,9 @@ __device__ __forceinline__ int find_segment(const tg_cg_seg *segs, int nseg, int
 }
 
 template <typename T>
-__device__ __forceinline__ T reg_value(const tg_cg_seg &g, const T *x, int64_t k) {
+__device__ __forceinline__ T reg_value(const tg_cg_seg &g, gptr<const T> x, int64_t k) {
   const int64_t n[4] = {g.shape[0], g.shape[1], g.shape[2], g.shape[3]};
-  return reg_at<T>(n, g.axes, g.identity, x, (const T *)g.weight, k);
+  return reg_at<T>(n, g.axes, g.identity, x, (gptr<const T>)g.weight, k);
 }
 
 template <typename T, int OP>
@@ -218,10 +256,13 @@ __global__ __launch_bounds__(256) void k_cg_seg(const tg_cg_seg *__restrict__ se
   const int sidx = find_segment(segs, nseg, blockIdx.x);
   const tg_cg_seg &g = segs[sidx];
   const int64_t base = ((int64_t)blockIdx.x - g.block0) * kSegElems;
-  T *x = (T *)g.x;
-  T *r = (T *)g.r;
-  T *p = (T *)g.p;
-  T *q = (T *)g.q;
+  // a solve's four vectors are distinct arrays: the loads of the unrolled
+  // elements below may be issued before the stores of earlier ones
+  gptr<T> __restrict__ x = (gptr<T>)g.x;
+  gptr<T> __restrict__ r = (gptr<T>)g.r;
+  gptr<T> __restrict__ p = (gptr<T>)g.p;
+  gptr<T> __restrict__ q = (gptr<T>)g.q;
+  gptr<const uint8_t> voxmask = (gptr<const uint8_t>)g.voxmask;
   double acc = 0.0;
   T coef = 0;
   if (OP == TG_CG_OP_UPDATE) {
@@ -231,11 +272,20 @@ __global__ __launch_bounds__(256) void k_cg_seg(const tg_cg_seg *__restrict__ se
     const double den = g.scal[0];
     coef = (T)(den != 0.0 ? g.scal[2] / den : 0.0);
   }
+#pragma unroll
   for (int it = 0; it < kSegElems / 256; it++) {
     const int64_t k = base + it * 256 + threadIdx.x;
-    if (k >= g.n) break;
+    if (k >= g.n) continue;
     if (OP == TG_CG_OP_REGULARIZE) {
-      if (!(g.voxmask && g.voxmask[k])) q[k] = q[k] + (T)g.lam * reg_value<T>(g, p, k);
+      if (!(voxmask && voxmask[k])) q[k] = q[k] + (T)g.lam * reg_value<T>(g, p, k);
+    } else if (OP == TG_CG_OP_REGULARIZE_DOT) {
+      // REGULARIZE then DOT_PQ in one pass, with the same arithmetic
+      T qk = q[k];
+      if (!(voxmask && voxmask[k])) {
+        qk = qk + (T)g.lam * reg_value<T>(g, p, k);
+        q[k] = qk;
+      }
+      acc += (double)p[k] * (double)qk;
     } else if (OP == TG_CG_OP_DOT_PQ) {
       acc += (double)p[k] * (double)q[k];
     } else if (OP == TG_CG_OP_UPDATE) {
@@ -251,7 +301,8 @@ __global__ __launch_bounds__(256) void k_cg_seg(const tg_cg_seg *__restrict__ se
       q[k] = (T)0;
     }
   }
-  if (OP == TG_CG_OP_DOT_PQ || OP == TG_CG_OP_UPDATE || OP == TG_CG_OP_INIT) {
+  if (OP == TG_CG_OP_DOT_PQ || OP == TG_CG_OP_REGULARIZE_DOT || OP == TG_CG_OP_UPDATE ||
+      OP == TG_CG_OP_INIT) {
     lds[threadIdx.x] = acc;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -264,24 +315,26 @@ __global__ __launch_bounds__(256) void k_cg_seg(const tg_cg_seg *__restrict__ se
 
 // One workgroup per segment: reduce its partials in a fixed order and update
 // the segment's scalars.
+constexpr int kFinalLanes = 1024;
+
 template <int OP>
-__global__ __launch_bounds__(256) void k_cg_seg_final(const tg_cg_seg *__restrict__ segs,
-                                                      const double *partials) {
-  __shared__ double lds[256];
+__global__ __launch_bounds__(kFinalLanes) void k_cg_seg_final(const tg_cg_seg *__restrict__ segs,
+                                                              const double *partials) {
+  __shared__ double lds[kFinalLanes];
   const tg_cg_seg &g = segs[blockIdx.x];
   const int64_t nb = (g.n + kSegElems - 1) / kSegElems;
   double s = 0.0;
   if (OP != TG_CG_OP_DIRECTION)
-    for (int64_t k = threadIdx.x; k < nb; k += 256) s += partials[g.block0 + k];
+    for (int64_t k = threadIdx.x; k < nb; k += kFinalLanes) s += partials[g.block0 + k];
   lds[threadIdx.x] = s;
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
+  for (int off = kFinalLanes / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
   double *sc = g.scal;
-  if (OP == TG_CG_OP_DOT_PQ) {
+  if (OP == TG_CG_OP_DOT_PQ || OP == TG_CG_OP_REGULARIZE_DOT) {
     sc[1] = lds[0];
   } else if (OP == TG_CG_OP_UPDATE) {
     sc[3] = sc[1] != 0.0 ? sc[0] / sc[1] : 0.0;
@@ -300,7 +353,8 @@ int launch_seg(const tg_cg_seg *segs, int nseg, int64_t nblocks, double *partial
   hipLaunchKernelGGL((k_cg_seg<T, OP>), dim3((uint32_t)nblocks), dim3(256), 0, s, segs, nseg,
                      partials);
   if (OP != TG_CG_OP_REGULARIZE && OP != TG_CG_OP_ZERO_Q)
-    hipLaunchKernelGGL((k_cg_seg_final<OP>), dim3((uint32_t)nseg), dim3(256), 0, s, segs, partials);
+    hipLaunchKernelGGL((k_cg_seg_final<OP>), dim3((uint32_t)nseg), dim3(kFinalLanes), 0, s, segs,
+                       partials);
   return hip_check(hipGetLastError(), "segmented cg launch");
 }
 
@@ -310,6 +364,8 @@ int dispatch_seg(int op, const tg_cg_seg *segs, int nseg, int64_t nblocks, doubl
   switch (op) {
     case TG_CG_OP_REGULARIZE: return launch_seg<T, TG_CG_OP_REGULARIZE>(segs, nseg, nblocks, partials, s);
     case TG_CG_OP_DOT_PQ: return launch_seg<T, TG_CG_OP_DOT_PQ>(segs, nseg, nblocks, partials, s);
+    case TG_CG_OP_REGULARIZE_DOT:
+      return launch_seg<T, TG_CG_OP_REGULARIZE_DOT>(segs, nseg, nblocks, partials, s);
     case TG_CG_OP_UPDATE: return launch_seg<T, TG_CG_OP_UPDATE>(segs, nseg, nblocks, partials, s);
     case TG_CG_OP_DIRECTION: return launch_seg<T, TG_CG_OP_DIRECTION>(segs, nseg, nblocks, partials, s);
     case TG_CG_OP_INIT: return launch_seg<T, TG_CG_OP_INIT>(segs, nseg, nblocks, partials, s);

@@ -29,26 +29,48 @@ __device__ __forceinline__ uint32_t udiv_rcp(uint32_t k, uint32_t d, double rd) 
   return rem < 0 ? q - 1u : (rem >= (int64_t)d ? q + 1u : q);
 }
 
-// Coordinates m of flat index k in a C-order array of shape n, strides st
-// (st[3] = 1).  Arrays of at most 2^32 elements divide in 32 bits through
-// reciprocals (the integer divisions, 64-bit ones above all, are emulated at
-// many times the cost).
-__device__ __forceinline__ void grid_pos(const int64_t n[4], const int64_t st[4], int64_t k,
-                                         int64_t m[4]) {
-  if (n[0] * st[0] <= (int64_t)0xffffffff) {
-    const uint32_t kk = (uint32_t)k, s0 = (uint32_t)st[0], s1 = (uint32_t)st[1], s2 = (uint32_t)st[2];
-    const uint32_t a = udiv_rcp(kk, s0, 1.0 / (double)s0), r0 = kk - a * s0;
-    const uint32_t b = udiv_rcp(r0, s1, 1.0 / (double)s1), r1 = r0 - b * s1;
-    const uint32_t c = udiv_rcp(r1, s2, 1.0 / (double)s2);
+// A C-order array of shape n: its strides and, for arrays of at most 2^32
+// elements, the divisors and reciprocals that turn a flat index into
+// coordinates with 32-bit arithmetic (the integer divisions, 64-bit ones above
+// all, are emulated at many times the cost).  Computed once per thread.
+struct GridDiv {
+  int64_t n[4], st[4];
+  uint32_t s[3];
+  double r[3];
+  bool small;
+};
+
+__device__ __forceinline__ GridDiv grid_div(const int64_t n[4]) {
+  GridDiv d;
+  for (int a = 0; a < 4; a++) d.n[a] = n[a];
+  d.st[3] = 1;
+  d.st[2] = n[3];
+  d.st[1] = n[2] * n[3];
+  d.st[0] = n[1] * n[2] * n[3];
+  d.small = n[0] * d.st[0] <= (int64_t)0xffffffff;
+  for (int a = 0; a < 3; a++) {
+    d.s[a] = (uint32_t)d.st[a];
+    d.r[a] = 1.0 / (double)(d.s[a] ? d.s[a] : 1u);
+  }
+  return d;
+}
+
+// Coordinates m of flat index k.
+__device__ __forceinline__ void grid_pos(const GridDiv &d, int64_t k, int64_t m[4]) {
+  if (d.small) {
+    const uint32_t kk = (uint32_t)k;
+    const uint32_t a = udiv_rcp(kk, d.s[0], d.r[0]), r0 = kk - a * d.s[0];
+    const uint32_t b = udiv_rcp(r0, d.s[1], d.r[1]), r1 = r0 - b * d.s[1];
+    const uint32_t c = udiv_rcp(r1, d.s[2], d.r[2]);
     m[0] = a;
     m[1] = b;
     m[2] = c;
-    m[3] = r1 - c * s2;
+    m[3] = r1 - c * d.s[2];
   } else {
-    m[0] = k / st[0];
-    m[1] = (k / st[1]) % n[1];
-    m[2] = (k / st[2]) % n[2];
-    m[3] = k % n[3];
+    m[0] = k / d.st[0];
+    m[1] = (k / d.st[1]) % d.n[1];
+    m[2] = (k / d.st[2]) % d.n[2];
+    m[3] = k % d.n[3];
   }
 }
 
@@ -57,34 +79,58 @@ __device__ __forceinline__ void grid_pos(const int64_t n[4], const int64_t st[4]
 template <typename T>
 using gptr = __attribute__((address_space(1))) T *;
 
-// (identity ? x_k : 0) + sum_k D_k^T D_k x at flat index k of an array of
-// shape n[4] (C order), or with the voxel weight w (not null)
+// (identity ? x_k : 0) + sum_k D_k^T D_k x at flat index k of the array d
+// describes, or with the voxel weight w (not null)
 // (identity ? x_k : 0) + w_k sum_k D_k^T D_k (w x).  P: the pointer type.
+// Every operand is loaded (a neighbour outside the array at the element
+// itself) before any is used, so the loads are in flight together; only the
+// sums depend on the bounds.
 template <typename T, typename P>
-__device__ __forceinline__ T reg_at(const int64_t n[4], uint32_t axes, int identity, P x, P w,
+__device__ __forceinline__ T reg_at(const GridDiv &d, uint32_t axes, int identity, P x, P w,
                                     int64_t k) {
-  const int64_t st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
   int64_t m[4];
-  grid_pos(n, st, k, m);
-  const T xc = x[k];
-  T acc = identity ? xc : (T)0;
+  grid_pos(d, k, m);
+  int64_t lo[4], hi[4];
+#pragma unroll
+  for (int ax = 0; ax < 4; ax++) {
+    lo[ax] = m[ax] >= 1 ? k - d.st[ax] : k;
+    hi[ax] = m[ax] <= d.n[ax] - 2 ? k + d.st[ax] : k;
+  }
   if (!w) {
+    const T xc = x[k];
+    T xl[4], xh[4];
+#pragma unroll
+    for (int ax = 0; ax < 4; ax++) {
+      xl[ax] = (axes & (1u << ax)) ? x[lo[ax]] : (T)0;
+      xh[ax] = (axes & (1u << ax)) ? x[hi[ax]] : (T)0;
+    }
+    T acc = identity ? xc : (T)0;
 #pragma unroll
     for (int ax = 0; ax < 4; ax++) {
       if (!(axes & (1u << ax))) continue;
-      if (m[ax] >= 1) acc += xc - x[k - st[ax]];
-      if (m[ax] <= n[ax] - 2) acc -= x[k + st[ax]] - xc;
+      if (m[ax] >= 1) acc += xc - xl[ax];
+      if (m[ax] <= d.n[ax] - 2) acc -= xh[ax] - xc;
     }
     return acc;
   }
-  const T wc = w[k];
+  const T xc = x[k], wc = w[k];
+  T xl[4], xh[4], wl[4], wh[4];
+#pragma unroll
+  for (int ax = 0; ax < 4; ax++) {
+    const bool on = (axes & (1u << ax)) != 0;
+    xl[ax] = on ? x[lo[ax]] : (T)0;
+    xh[ax] = on ? x[hi[ax]] : (T)0;
+    wl[ax] = on ? w[lo[ax]] : (T)0;
+    wh[ax] = on ? w[hi[ax]] : (T)0;
+  }
+  const T acc = identity ? xc : (T)0;
   const T uc = wc * xc;
   T dd = (T)0;
 #pragma unroll
   for (int ax = 0; ax < 4; ax++) {
     if (!(axes & (1u << ax))) continue;
-    if (m[ax] >= 1) dd += uc - w[k - st[ax]] * x[k - st[ax]];
-    if (m[ax] <= n[ax] - 2) dd -= w[k + st[ax]] * x[k + st[ax]] - uc;
+    if (m[ax] >= 1) dd += uc - wl[ax] * xl[ax];
+    if (m[ax] <= d.n[ax] - 2) dd -= wh[ax] * xh[ax] - uc;
   }
   return acc + wc * dd;
 }
@@ -98,10 +144,11 @@ __global__ __launch_bounds__(256) void k_regularize(int64_t n0, int64_t n1, int6
                                                     const T *__restrict__ w) {
   const int64_t total = n0 * n1 * n2 * n3;
   const int64_t n[4] = {n0, n1, n2, n3};
+  const GridDiv gd = grid_div(n);
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
        k += (int64_t)gridDim.x * blockDim.x) {
     if (mask && mask[k]) continue;
-    out[k] = out[k] + lam * reg_at<T>(n, axes, identity, x, w, k);
+    out[k] = out[k] + lam * reg_at<T>(gd, axes, identity, x, w, k);
   }
 }
 
@@ -117,31 +164,36 @@ __global__ __launch_bounds__(256) void k_smooth(int64_t n0, int64_t n1, int64_t 
                                                 double *scal, const T *__restrict__ vw) {
   __shared__ double lds[256];
   const int64_t total = n0 * n1 * n2 * n3;
-  const int64_t s3 = 1, s2 = n3, s1 = n2 * n3, s0 = n1 * n2 * n3;
+  const int64_t n[4] = {n0, n1, n2, n3};
+  const GridDiv gd = grid_div(n);
+  const int64_t *st = gd.st;
   const double w[4] = {w0, w1, w2, w3};
   double v = 0.0;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
        k += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t n[4] = {n0, n1, n2, n3};
-    const int64_t st[4] = {s0, s1, s2, s3};
     int64_t m[4];
-    grid_pos(n, st, k, m);
+    grid_pos(gd, k, m);
     const T xc = vw ? vw[k] * x[k] : x[k];
+    // both neighbours of every axis loaded at once (at k where outside)
+    T un[4], up[4];
+#pragma unroll
+    for (int ax = 0; ax < 4; ax++) {
+      const int64_t kn = m[ax] <= n[ax] - 2 ? k + st[ax] : k;
+      const int64_t kp = m[ax] >= 1 ? k - st[ax] : k;
+      un[ax] = w[ax] != 0.0 ? (vw ? vw[kn] * x[kn] : x[kn]) : (T)0;
+      up[ax] = w[ax] != 0.0 ? (vw ? vw[kp] * x[kp] : x[kp]) : (T)0;
+    }
     T g = (T)0;
 #pragma unroll
     for (int ax = 0; ax < 4; ax++) {
       if (w[ax] == 0.0) continue;
       const T wt = (T)w[ax];
       if (m[ax] <= n[ax] - 2) {  // the difference x[m + e] - x[m] belongs to m
-        const int64_t kn = k + st[ax];
-        const T d = (vw ? vw[kn] * x[kn] : x[kn]) - xc;
+        const T d = un[ax] - xc;
         v += w[ax] * ((double)d * (double)d);
         g -= wt * d;
       }
-      if (m[ax] >= 1) {
-        const int64_t kp = k - st[ax];
-        g += wt * (xc - (vw ? vw[kp] * x[kp] : x[kp]));
-      }
+      if (m[ax] >= 1) g += wt * (xc - up[ax]);
     }
     if (grad) grad[k] = vw ? (T)2 * (vw[k] * g) : (T)2 * g;
   }
@@ -243,10 +295,15 @@ __device__ __forceinline__ int find_segment(const tg_cg_seg *segs, int nseg, int
   return lo;
 }
 
-template <typename T>
-__device__ __forceinline__ T reg_value(const tg_cg_seg &g, gptr<const T> x, int64_t k) {
+__device__ __forceinline__ GridDiv seg_grid(const tg_cg_seg &g) {
   const int64_t n[4] = {g.shape[0], g.shape[1], g.shape[2], g.shape[3]};
-  return reg_at<T>(n, g.axes, g.identity, x, (gptr<const T>)g.weight, k);
+  return grid_div(n);
+}
+
+template <typename T>
+__device__ __forceinline__ T reg_value(const tg_cg_seg &g, const GridDiv &gd, gptr<const T> x,
+                                       int64_t k) {
+  return reg_at<T>(gd, g.axes, g.identity, x, (gptr<const T>)g.weight, k);
 }
 
 template <typename T, int OP>
@@ -263,6 +320,8 @@ __global__ __launch_bounds__(256) void k_cg_seg(const tg_cg_seg *__restrict__ se
   gptr<T> __restrict__ p = (gptr<T>)g.p;
   gptr<T> __restrict__ q = (gptr<T>)g.q;
   gptr<const uint8_t> voxmask = (gptr<const uint8_t>)g.voxmask;
+  GridDiv gd;
+  if (OP == TG_CG_OP_REGULARIZE || OP == TG_CG_OP_REGULARIZE_DOT) gd = seg_grid(g);
   double acc = 0.0;
   T coef = 0;
   if (OP == TG_CG_OP_UPDATE) {
@@ -277,12 +336,12 @@ __global__ __launch_bounds__(256) void k_cg_seg(const tg_cg_seg *__restrict__ se
     const int64_t k = base + it * 256 + threadIdx.x;
     if (k >= g.n) continue;
     if (OP == TG_CG_OP_REGULARIZE) {
-      if (!(voxmask && voxmask[k])) q[k] = q[k] + (T)g.lam * reg_value<T>(g, p, k);
+      if (!(voxmask && voxmask[k])) q[k] = q[k] + (T)g.lam * reg_value<T>(g, gd, p, k);
     } else if (OP == TG_CG_OP_REGULARIZE_DOT) {
       // REGULARIZE then DOT_PQ in one pass, with the same arithmetic
       T qk = q[k];
       if (!(voxmask && voxmask[k])) {
-        qk = qk + (T)g.lam * reg_value<T>(g, p, k);
+        qk = qk + (T)g.lam * reg_value<T>(g, gd, p, k);
         q[k] = qk;
       }
       acc += (double)p[k] * (double)qk;

@@ -98,7 +98,12 @@ class Launch:
     to its output after it.
     """
 
-    def __init__(self, ops, ins, outs, backward=False, zero_out=False):
+    def __init__(self, ops, ins, outs, backward=False, zero_out=False, packs=None):
+        """packs: optional per-job dicts that keep each job's packed call between
+        launches over the same operands (a CG solve's fixed buffers), so that a
+        new set of jobs reuses the calls of the jobs it shares with an earlier
+        one.  A job whose operator repeats an earlier job's in this batch is
+        packed afresh (it needs accumulators of its own)."""
         n = len(ops)
         self.n = n
         self.backward = bool(backward)
@@ -110,47 +115,34 @@ class Launch:
         self.prepass = []    # (operator, map, weighted-map buffer): forward voxel operands
         self.alive = []      # every tensor the packed calls point to
         calls = (N.TgCall * n)()
-        weighted = set()
+        seen = set()
         accs = set()
         targets = set()
         for k, (P, x, y) in enumerate(zip(ops, ins, outs)):
             if y.data_ptr() in targets:
                 raise ValueError("job %d: a batch cannot list one output twice" % k)
             targets.add(y.data_ptr())
-            if self.backward:
-                target = y
-                if P.vox_weight is not None:
-                    target = torch.zeros_like(y)
-                    self.post.append((P, target, y))
-                data = self._operand(P._as_stack(x), x)
-                c = P._call(N.TG_BPJ, target, data)
-                if c.flags & N.TG_ACC_F64:
-                    if c.acc in accs:
-                        # the operator's float64 accumulator serves its first job;
-                        # a repeat of the operator sums into a buffer of its own
-                        own = torch.zeros_like(P.acc_buffer())
-                        self.alive.append(own)
-                        c.acc = own.data_ptr()
-                    accs.add(c.acc)
-            else:
-                target = y
-                if P.pix_weight is not None and y.stride() != P.pix_weight.stride():
-                    # pixel weights live in the operator's view-major layout
-                    target = P._as_stack(y)
-                    self.copies.append((target, y))
-                cube = self._operand(P._as_cube(x), x)
-                if P.vox_weight is not None or P.voxmask is not None:
-                    # an operator's weighted-map buffer serves its first job only
-                    wmap = P.vox_buffer(torch.empty_like(P.xin) if id(P) in weighted else None)
-                    weighted.add(id(P))
-                    self.prepass.append((P, cube, wmap))
-                    self.alive.append(wmap)
-                    cube = wmap
-                c = P._call(N.TG_PJ, cube, target, apply_vox=False)
-                if self.zero_out:
-                    c.flags |= N.TG_ZERO_OUT
-            self.alive += [target, P]
-            ctypes.memmove(ctypes.byref(calls[k]), ctypes.byref(c), ctypes.sizeof(N.TgCall))
+            key = (self.backward, self.zero_out, x.data_ptr(), tuple(x.shape), tuple(x.stride()),
+                   y.data_ptr(), tuple(y.shape), tuple(y.stride()))
+            first = id(P) not in seen
+            seen.add(id(P))
+            pack = packs[k].get(key) if (packs is not None and first) else None
+            if pack is None:
+                pack = self._pack(P, x, y, first)
+                if packs is not None and first:
+                    packs[k][key] = pack
+            if self.backward and pack[0].flags & N.TG_ACC_F64 and pack[0].acc in accs:
+                # another operator's job sums into this accumulator: one of its own
+                pack = self._pack(P, x, y, False)
+            call, refresh, copies, post, prepass, alive = pack
+            if self.backward and call.flags & N.TG_ACC_F64:
+                accs.add(call.acc)
+            self.refresh += refresh
+            self.copies += copies
+            self.post += post
+            self.prepass += prepass
+            self.alive += alive + [P]
+            ctypes.memmove(ctypes.byref(calls[k]), ctypes.byref(call), ctypes.sizeof(N.TgCall))
         self.calls = calls
         self.pcalls = self.plans = self.acalls = None
         if self.backward:
@@ -172,13 +164,52 @@ class Launch:
                     ctypes.memmove(ctypes.byref(self.acalls[q]), ctypes.byref(calls[k]),
                                    ctypes.sizeof(N.TgCall))
 
-    def _operand(self, t, x):
-        """The operator-layout operand t of the caller's x (a copy when layouts,
-        devices or dtypes differ, then refreshed on every run)."""
-        self.alive.append(t)
-        if not isinstance(x, torch.Tensor) or t.data_ptr() != x.data_ptr():
-            self.refresh.append((t, x))
-        return t
+    def _pack(self, P, x, y, first):
+        """One job's packed call and what its runs need: (call, refresh, copies,
+        post, prepass, alive), as the lists of the same names above.  `first`:
+        P has no earlier job in this batch (it then uses its own float64
+        accumulator and weighted-map buffer; a repeat gets fresh ones)."""
+        refresh, copies, post, prepass, alive = [], [], [], [], []
+
+        def operand(t, xx):
+            # the operator-layout operand t of the caller's xx (a copy when
+            # layouts, devices or dtypes differ, then refreshed on every run)
+            alive.append(t)
+            if not isinstance(xx, torch.Tensor) or t.data_ptr() != xx.data_ptr():
+                refresh.append((t, xx))
+            return t
+
+        if self.backward:
+            target = y
+            if P.vox_weight is not None:
+                target = torch.zeros_like(y)
+                post.append((P, target, y))
+            data = operand(P._as_stack(x), x)
+            c = P._call(N.TG_BPJ, target, data)
+            if (c.flags & N.TG_ACC_F64) and not first:
+                # the operator's float64 accumulator serves its first job; a
+                # repeat of the operator sums into a buffer of its own
+                own = torch.zeros_like(P.acc_buffer())
+                alive.append(own)
+                c.acc = own.data_ptr()
+        else:
+            target = y
+            if P.pix_weight is not None and y.stride() != P.pix_weight.stride():
+                # pixel weights live in the operator's view-major layout
+                target = P._as_stack(y)
+                copies.append((target, y))
+            cube = operand(P._as_cube(x), x)
+            if P.vox_weight is not None or P.voxmask is not None:
+                # an operator's weighted-map buffer serves its first job only
+                wmap = P.vox_buffer(None if first else torch.empty_like(P.xin))
+                prepass.append((P, cube, wmap))
+                alive.append(wmap)
+                cube = wmap
+            c = P._call(N.TG_PJ, cube, target, apply_vox=False)
+            if self.zero_out:
+                c.flags |= N.TG_ZERO_OUT
+        alive.append(target)
+        return c, refresh, copies, post, prepass, alive
 
     def run(self, plan=None):
         if self.n == 0:

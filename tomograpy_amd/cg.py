@@ -66,6 +66,8 @@ class CGSolve:
         self.weight = _prior_weight(P, axes, weight, self.store_shape)
         self.y = P._as_stack(y)
         self.out = P.xout  # forward output buffer of this solve
+        # the solve's packed batched-launch calls (its operands never change)
+        self.packs = {}
         # r0 = P^T y (x0 = 0), p0 = r0
         P.backproject_into(self.y, self.view(self.rs))
         self.ps.copy_(self.rs)
@@ -177,8 +179,9 @@ def iterate(solves, table, tuner=None):
             qs = [s.view(s.qs) for s in solves]
             plan_f = tuner.plan_for(ops, ins, outs, False, zero_out=True) if tuner else None
             plan_b = tuner.plan_for(ops, outs, qs, True) if tuner else None
-            table.launches = (B.Launch(ops, ins, outs, backward=False, zero_out=True), plan_f,
-                              B.Launch(ops, outs, qs, backward=True), plan_b)
+            packs = [s.packs for s in solves]
+            table.launches = (B.Launch(ops, ins, outs, backward=False, zero_out=True, packs=packs),
+                              plan_f, B.Launch(ops, outs, qs, backward=True, packs=packs), plan_b)
         fwd, plan_f, bwd, plan_b = table.launches
         fwd.run(plan_f)
         # geometries with a brick plan backproject in one plan launch, the

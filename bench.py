@@ -790,9 +790,17 @@ def main():
             ref32["cg_x"] = keep["x"]
             # the same solve with the axis views' pixels masked out
             Pm = siddon_lo(hs, ch, dtype=dtype, mask=view_mask(hs, args.pix, args.pix, av))
-            cg_run(Pm, y, args.cg_iters, keep)
+            masked = cg_run(Pm, y, args.cg_iters, keep)
             ref32["cg_x_other"] = keep["x"]
             del Pm
+            if av:
+                f32 = extra["cg"]["fp32"]
+                f32["relative_residual_axis_views_masked"] = masked["relative_residual"]
+                f32["note"] = ("float32 CG stalls near a residual of 0.1 because views %s put the "
+                               "observer on a coordinate plane, where the reference's float32 ray "
+                               "set-up gives a few rays steps of about +-212 (error_vs_fp64); with "
+                               "those views' pixels masked it reaches the residual above; float64 "
+                               "does not stall" % av)
 
     y_host = y.cpu().numpy() if not args.no_cpu else None
     if args.serve_requests or args.serve_small_requests or not args.no_grad \

@@ -226,6 +226,31 @@ def _mask_tensor(mask, shape, dev):
     return (m != 0).to(torch.uint8)
 
 
+# A backprojection call of at least this many steps (bounded by rays x
+# (n0 + n1 + n2)) goes through a brick plan built for the call -- count, fill,
+# pass sort, then the brick kernel -- instead of one global atomic per step.
+# Measured per call, build included, on device arrays: 256^3 seen by 128
+# views of 512^2 in 31.5 against 155 ms; 128^3 by 64 of 512^2 in 9.0 against
+# 79 ms; 128^3 by 64 of 256^2 in 2.8 against 10.9 ms.  Near 10^8 the two are
+# alike (96^3 by 32 of 128^2: 0.81 against 0.74 ms).
+PLAN_CALL_MIN_STEPS = 1 << 28
+
+
+def _plan_backprojection(data_header, cube_header, d, c, mask, obstacle, tdtype, dev):
+    """cube += P^T data through a brick plan of this geometry built for the
+    call (the operators' backward: float64 sums, order-independent).  Returns
+    False, with nothing done, when the plan would take more than half the free
+    device memory."""
+    from .lo_wrapper import Siddon
+    P = Siddon(data_header, cube_header, mask=mask, obstacle=obstacle, dtype=tdtype, device=dev,
+               bpj_mode="plan")
+    free, _ = torch.cuda.mem_get_info(dev)
+    if P.plan_reserve_bytes() > free // 2:
+        return False
+    P.backproject_into(d.t, c.t)
+    return True
+
+
 def _siddon(kind, data, cube, mask=None, obstacle=None, t=None, ray=None, deterministic=False,
             tindex=None, tile_j=0, sync=None):
     if obstacle not in _OBSTACLES:
@@ -266,24 +291,30 @@ def _siddon(kind, data, cube, mask=None, obstacle=None, t=None, ray=None, determ
     bwd = kind in (N.TG_BPJ, N.TG_BPJT)
     if deterministic and bwd:
         flags |= N.TG_DETERMINISTIC
-    st = _status_tensor(dev)
-    call = make_call(kind, code, d.t.shape, d.t.stride(), c.t.shape, c.t.stride(),
-                     map_record(cube_header), views, c.t.data_ptr(), d.t.data_ptr(), flags=flags,
-                     nt=nt, tile_j=tile_j, pixmask=pm, tindex=ti, status=st)
     ws = None
-    if flags & N.TG_DETERMINISTIC:
-        ws = det_workspace(call, dev)
-        call.workspace = ws.data_ptr()
-        call.workspace_bytes = ws.numel()
-    lib = N.lib()
-    s = stream_handle()
-    if ray is not None:
-        i, j, tt = (int(x) for x in ray)
-        N.check(lib.tg_ray_projector(ctypes.byref(call), i, j, tt, s))
-    elif t is not None:
-        N.check(lib.tg_conic_image_projector(ctypes.byref(call), int(t), s))
-    else:
-        N.check(lib.tg_conic_full_projector(ctypes.byref(call), s))
+    planned = (kind == N.TG_BPJ and not deterministic and t is None and ray is None
+               and n1 * n2 * nv * sum(c.t.shape) >= PLAN_CALL_MIN_STEPS
+               and n1 * n2 * nv < (1 << 32) and max(c.t.shape) <= 1024
+               and _plan_backprojection(data_header, cube_header, d, c, mask, obstacle, tdtype,
+                                        dev))
+    if not planned:
+        st = _status_tensor(dev)
+        call = make_call(kind, code, d.t.shape, d.t.stride(), c.t.shape, c.t.stride(),
+                         map_record(cube_header), views, c.t.data_ptr(), d.t.data_ptr(),
+                         flags=flags, nt=nt, tile_j=tile_j, pixmask=pm, tindex=ti, status=st)
+        if flags & N.TG_DETERMINISTIC:
+            ws = det_workspace(call, dev)
+            call.workspace = ws.data_ptr()
+            call.workspace_bytes = ws.numel()
+        lib = N.lib()
+        s = stream_handle()
+        if ray is not None:
+            i, j, tt = (int(x) for x in ray)
+            N.check(lib.tg_ray_projector(ctypes.byref(call), i, j, tt, s))
+        elif t is not None:
+            N.check(lib.tg_conic_image_projector(ctypes.byref(call), int(t), s))
+        else:
+            N.check(lib.tg_conic_full_projector(ctypes.byref(call), s))
     host = d.host is not None or c.host is not None
     if sync or (sync is None and host):
         torch.cuda.current_stream().synchronize()

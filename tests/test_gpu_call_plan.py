@@ -106,3 +106,26 @@ def test_small_call_keeps_atomic_scatter(spy):
     T.backprojector(T.InfoArray(y, header=hs), out)
     assert spy == []
     assert np.abs(np.asarray(out)).sum() > 0
+
+
+@pytest.mark.parametrize("explicit_bins", [False, True])
+def test_large_4d_call_through_plan_matches_oracle(oracle, spy, explicit_bins):
+    """backprojector4d: image t into bin t, or tindex[t] (C_siddon.c.template:931-944)."""
+    print("call plan 4d", explicit_bins, flush=True)
+    ch, hs = geometry(np.float32)
+    nv = len(hs)
+    rng = np.random.default_rng(35)
+    y = rng.uniform(-1, 1, (128, 128, nv)).astype(np.float32)
+    if explicit_bins:
+        tindex = (np.arange(nv) * 5) % 3
+        nt = 3
+    else:
+        tindex, nt = None, nv
+    start = rng.uniform(0, 1, (96, 96, 96, nt)).astype(np.float32)
+    ref = start.copy()
+    oracle.siddon(oracle.BPJT, view_records(hs), map_record(ch), ref, T.InfoArray(y, header=hs),
+                  nthread=1, tindex=tindex)
+    cube = T.InfoArray(start.copy(), header=dict(ch))
+    T.backprojector4d(T.InfoArray(y, header=hs), cube, tindex=tindex)
+    assert spy == [True]
+    assert rel_l2(cube, ref) <= 1e-5

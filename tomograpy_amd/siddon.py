@@ -236,14 +236,23 @@ def _mask_tensor(mask, shape, dev):
 PLAN_CALL_MIN_STEPS = 1 << 28
 
 
-def _plan_backprojection(data_header, cube_header, d, c, mask, obstacle, tdtype, dev):
+def _plan_backprojection(data_header, cube_header, d, c, mask, obstacle, tdtype, dev,
+                         tindex=None):
     """cube += P^T data through a brick plan of this geometry built for the
-    call (the operators' backward: float64 sums, order-independent).  Returns
-    False, with nothing done, when the plan would take more than half the free
-    device memory."""
-    from .lo_wrapper import Siddon
-    P = Siddon(data_header, cube_header, mask=mask, obstacle=obstacle, dtype=tdtype, device=dev,
-               bpj_mode="plan")
+    call (the operators' backward: float32 data sums in float64, independent
+    of the order).  A 4-D cube (N0, N1, N2, T) takes image t into bin
+    tindex[t] (default t).  Returns False, with nothing done, when the plan
+    would take more than half the free device memory."""
+    from .lo_wrapper import Siddon, Siddon4d
+    if c.t.dim() == 4:
+        hdr = dict(cube_header)
+        hdr["NAXIS4"] = int(c.t.shape[3])
+        ti = np.arange(len(data_header)) if tindex is None else np.asarray(tindex)
+        P = Siddon4d(data_header, hdr, tindex=ti, mask=mask, obstacle=obstacle, dtype=tdtype,
+                     device=dev, bpj_mode="plan")
+    else:
+        P = Siddon(data_header, cube_header, mask=mask, obstacle=obstacle, dtype=tdtype,
+                   device=dev, bpj_mode="plan")
     free, _ = torch.cuda.mem_get_info(dev)
     if P.plan_reserve_bytes() > free // 2:
         return False
@@ -292,11 +301,11 @@ def _siddon(kind, data, cube, mask=None, obstacle=None, t=None, ray=None, determ
     if deterministic and bwd:
         flags |= N.TG_DETERMINISTIC
     ws = None
-    planned = (kind == N.TG_BPJ and not deterministic and t is None and ray is None
-               and n1 * n2 * nv * sum(c.t.shape) >= PLAN_CALL_MIN_STEPS
-               and n1 * n2 * nv < (1 << 32) and max(c.t.shape) <= 1024
+    planned = (bwd and not deterministic and t is None and ray is None
+               and n1 * n2 * nv * sum(c.t.shape[:3]) >= PLAN_CALL_MIN_STEPS
+               and n1 * n2 * nv < (1 << 32) and max(c.t.shape[:3]) <= 1024
                and _plan_backprojection(data_header, cube_header, d, c, mask, obstacle, tdtype,
-                                        dev))
+                                        dev, tindex))
     if not planned:
         st = _status_tensor(dev)
         call = make_call(kind, code, d.t.shape, d.t.stride(), c.t.shape, c.t.stride(),

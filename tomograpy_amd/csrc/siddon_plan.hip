@@ -715,9 +715,16 @@ __device__ __forceinline__ void sort8(uint32_t (&v)[8]) {
   cx(1, 2); cx(3, 4); cx(5, 6);
 }
 
+// The bucket words of the deterministic kernel are padded by one word per 8
+// cells: the bucket scans read each lane's 8 consecutive cells, and with the
+// padding the 64 lanes' words of one read fall in different banks instead of
+// 16 to a bank.
+__host__ __device__ __forceinline__ int det_cnt_index(int c) { return c + (c >> 3); }
+__host__ __device__ __forceinline__ int det_cnt_words(int cells) { return cells + (cells >> 3) + 1; }
+
 template <typename T>
 __host__ __device__ __forceinline__ size_t det_lds_bytes(int cells, int mcap) {
-  return (size_t)cells * sizeof(T) + (size_t)mcap * sizeof(T) + (size_t)cells * 4 +
+  return (size_t)cells * sizeof(T) + (size_t)mcap * sizeof(T) + (size_t)det_cnt_words(cells) * 4 +
          4 * (kBpjBlock / 64 + 4) + (size_t)mcap * 4;
 }
 
@@ -731,7 +738,7 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
   T *acc = reinterpret_cast<T *>(smem_raw);
   T *uval = acc + cells;
   uint32_t *cnt = reinterpret_cast<uint32_t *>(uval + mcap);
-  uint32_t *wsum = cnt + cells;  // kBpjBlock / 64 + 1 words, then the batch's record and slot counts
+  uint32_t *wsum = cnt + det_cnt_words(cells);  // kBpjBlock / 64 + 1 words, then the batch's counts
   uint32_t *batch = wsum + (kBpjBlock / 64 + 1);
   uint16_t *ucell = reinterpret_cast<uint16_t *>(batch + 3);
   uint16_t *sidx = ucell + mcap;
@@ -749,7 +756,7 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
     const int64_t i0 = lo0 + l0, i1 = lo1 + l1, i2 = lo2 + l2;
     const bool inside = i0 < (int64_t)a.n[0] && i1 < (int64_t)a.n[1] && i2 < (int64_t)a.n[2];
     acc[c] = inside ? a.cube[boff + i0 * a.cs[0] + i1 * a.cs[1] + i2 * a.cs[2]] : (T)0;
-    cnt[c] = 0u;
+    cnt[det_cnt_index(c)] = 0u;
   }
   const bool view_major = a.ds[1] == 1 && a.ds[0] == (int64_t)a.n2 &&
                           a.ds[2] == (int64_t)a.n1 * (int64_t)a.n2;
@@ -820,7 +827,7 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
         D2 = sz ? e.pabs[2] : D2 - d;
         ucell[m] = (uint16_t)cell;
         uval[m] = d * w;
-        atomicAdd(&cnt[cell], 1u);
+        atomicAdd(&cnt[det_cnt_index(cell)], 1u);
         cell += (sx ? u0 * dm0 : 0) + (sy ? u1 * dm1 : 0) + (sz ? u2 : 0);
       }
     }
@@ -831,15 +838,15 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
     const int c0 = (int)tid * per;
     uint32_t local = 0;
     for (int c = c0; c < c0 + per && c < cells; c++) {
-      const uint32_t k = cnt[c];
+      const uint32_t k = cnt[det_cnt_index(c)];
       local += (k << 16) | (k != 0u ? 1u : 0u);
     }
     const uint32_t excl = block_incl_scan(local, wsum) - local;
     const uint32_t nact = wsum[kBpjBlock / 64] & 0xffffu;
     uint32_t run = excl >> 16;
     for (int c = c0; c < c0 + per && c < cells; c++) {
-      const uint32_t k = cnt[c];
-      cnt[c] = k ? (run << 16) | k : 0u;
+      const uint32_t k = cnt[det_cnt_index(c)];
+      cnt[det_cnt_index(c)] = k ? (run << 16) | k : 0u;
       run += k;
     }
     __syncthreads();
@@ -853,7 +860,8 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
       }
 #pragma unroll
       for (int j = 0; j < 4; j++)
-        if (cs[j] != 0xffffffffu) pos[j] = atomicAdd(&cnt[cs[j]], 1u << 16) >> 16;
+        if (cs[j] != 0xffffffffu)
+          pos[j] = atomicAdd(&cnt[det_cnt_index((int)cs[j])], 1u << 16) >> 16;
 #pragma unroll
       for (int j = 0; j < 4; j++)
         if (cs[j] != 0xffffffffu) sidx[pos[j]] = (uint16_t)(m0 + j * kBpjBlock);
@@ -867,7 +875,7 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
     // the cells with contributions, listed over the slots' cells (nact <= nslot)
     uint32_t ap = excl & 0xffffu;
     for (int c = c0; c < c0 + per && c < cells; c++)
-      if (cnt[c] & 0xffffu) ucell[ap++] = (uint16_t)c;
+      if (cnt[det_cnt_index(c)] & 0xffffu) ucell[ap++] = (uint16_t)c;
     __syncthreads();
     // each listed cell adds its bucket in slot order, one contribution at a
     // time, and leaves its bucket word zero for the next batch.  The slot
@@ -875,8 +883,8 @@ __global__ __launch_bounds__(kBpjBlock) void k_plan_bpj_det(KArgs<T> a, BrickGeo
     // network), longer buckets in place.
     for (uint32_t q = tid; q < nact; q += kBpjBlock) {
       const int c = ucell[q];
-      const uint32_t v = cnt[c];
-      cnt[c] = 0u;
+      const uint32_t v = cnt[det_cnt_index(c)];
+      cnt[det_cnt_index(c)] = 0u;
       const uint32_t k = v & 0xffffu;
       const uint32_t end = v >> 16, st = end - k;
       T sum = acc[c];

@@ -1315,12 +1315,14 @@ int launch_plan_bpj(const KArgs<T> &a, const BrickGeom &g, const tg_bpj_plan *p,
   return hip_check(hipGetLastError(), "plan bpj launch");
 }
 
-// Slots of contributions per batch of the deterministic kernel: the LDS left
-// by the brick's running sums and bucket words.
+// Slots of contributions per batch of the deterministic kernel: as many as
+// the LDS left by the brick's running sums and bucket words holds (a multiple
+// of 256, at most 16384).  Each batch scans every bucket word once, so the
+// more slots a batch takes, the less that scan costs per contribution.
 template <typename T>
 int det_slots(int cells) {
-  int mcap = 8192;
-  while (mcap > kDetMinSlots && det_lds_bytes<T>(cells, mcap) > (size_t)kPlanLdsBytes) mcap >>= 1;
+  int mcap = 16384;
+  while (mcap > kDetMinSlots && det_lds_bytes<T>(cells, mcap) > (size_t)kPlanLdsBytes) mcap -= 256;
   return mcap;
 }
 

@@ -228,11 +228,14 @@ def _mask_tensor(mask, shape, dev):
 
 # A backprojection call of at least this many steps (bounded by rays x
 # (n0 + n1 + n2)) goes through a brick plan built for the call -- count, fill,
-# pass sort, then the brick kernel -- instead of one global atomic per step.
-# Measured per call, build included, on device arrays: 256^3 seen by 128
-# views of 512^2 in 31.5 against 155 ms; 128^3 by 64 of 512^2 in 9.0 against
-# 79 ms; 128^3 by 64 of 256^2 in 2.8 against 10.9 ms.  Near 10^8 the two are
-# alike (96^3 by 32 of 128^2: 0.81 against 0.74 ms).
+# then the brick kernel -- instead of one global atomic per step.  Measured
+# per call, build included, on device arrays: 256^3 seen by 128 views of 512^2
+# in 26.2 against 155 ms; 128^3 by 64 of 512^2 in 8.7 against 79 ms; 128^3 by
+# 64 of 256^2 in 2.7 against 10.9 ms.  Near 10^8 the two are alike (96^3 by 32
+# of 128^2: 0.92 against 0.73 ms).  A plan used once is left in trace order:
+# sorting its passes by step count (what the operators do for the many
+# backprojections of a solve) costs more than it saves in one (at 256^3,
+# 128 x 512^2: 35.5 ms sorted against 29.9 ms per call).
 PLAN_CALL_MIN_STEPS = 1 << 28
 
 
@@ -249,10 +252,10 @@ def _plan_backprojection(data_header, cube_header, d, c, mask, obstacle, tdtype,
         hdr["NAXIS4"] = int(c.t.shape[3])
         ti = np.arange(len(data_header)) if tindex is None else np.asarray(tindex)
         P = Siddon4d(data_header, hdr, tindex=ti, mask=mask, obstacle=obstacle, dtype=tdtype,
-                     device=dev, bpj_mode="plan")
+                     device=dev, bpj_mode="plan", plan_order=0)
     else:
         P = Siddon(data_header, cube_header, mask=mask, obstacle=obstacle, dtype=tdtype,
-                   device=dev, bpj_mode="plan")
+                   device=dev, bpj_mode="plan", plan_order=0)
     free, _ = torch.cuda.mem_get_info(dev)
     if P.plan_reserve_bytes() > free // 2:
         return False

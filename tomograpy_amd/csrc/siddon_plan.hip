@@ -496,17 +496,35 @@ __global__ __launch_bounds__(kBpjBlock) __attribute__((amdgpu_waves_per_eu(8))) 
   chunk_range(g, offsets, chunk_prefix, blockIdx.x, r0, r1);
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
+  // each lane's record of the next pass, loaded while this pass is sorted (the
+  // passes are disjoint ranges, each rewritten in place)
+  uint32_t n_rid = 0, n_lq = 0;
+  T n_d0 = 0, n_d1 = 0, n_d2 = 0;
+  if (r0 + tid < r1) {
+    n_rid = rec.rid[r0 + tid];
+    n_lq = rec.lq[r0 + tid];
+    n_d0 = rec.d0[r0 + tid];
+    n_d1 = rec.d1[r0 + tid];
+    n_d2 = rec.d2[r0 + tid];
+  }
   for (unsigned long long ws = r0; ws < r1; ws += kBpjBlock) {
     const unsigned long long q = ws + tid;
     const bool valid = q < r1;
     uint32_t steps = 0;
     if (valid) {
-      s_rid[tid] = rec.rid[q];
-      s_lq[tid] = rec.lq[q];
-      s_d0[tid] = rec.d0[q];
-      s_d1[tid] = rec.d1[q];
-      s_d2[tid] = rec.d2[q];
-      steps = s_lq[tid] >> 16;
+      s_rid[tid] = n_rid;
+      s_lq[tid] = n_lq;
+      s_d0[tid] = n_d0;
+      s_d1[tid] = n_d1;
+      s_d2[tid] = n_d2;
+      steps = n_lq >> 16;
+    }
+    if (q + kBpjBlock < r1) {
+      n_rid = rec.rid[q + kBpjBlock];
+      n_lq = rec.lq[q + kBpjBlock];
+      n_d0 = rec.d0[q + kBpjBlock];
+      n_d1 = rec.d1[q + kBpjBlock];
+      n_d2 = rec.d2[q + kBpjBlock];
     }
     // longest first; ties by pass position with its low g.spread bits
     // reversed: spread, the walks a wavefront takes together come from rays

@@ -205,8 +205,9 @@ typedef struct tg_bpj_plan {
                             0 = trace order).  Spreading them puts rays that are not
                             neighbours in one wavefront, which pays when a voxel spans
                             several pixels: adjacent rays then add into the same LDS cells */
-  int32_t pairs_ready;   /* set by tg_bpj_plan_count: 1 when it left each trace workgroup's
-                            per-brick record counts in scratch for tg_bpj_plan_build */
+  int32_t pairs_ready;   /* set by tg_bpj_plan_count for tg_bpj_plan_build: bit 0 when it left
+                            each trace workgroup's per-brick record counts in scratch, bit 1
+                            when it wrote every ray's step count to ray_steps_out */
   int64_t records;       /* set by tg_bpj_plan_count */
   int64_t chunks;        /* set by tg_bpj_plan_count */
   void *meta;            /* device, caller-owned, meta_bytes */
@@ -220,7 +221,10 @@ typedef struct tg_bpj_plan {
   size_t scratch_bytes;  /* size that allows it, set by tg_bpj_plan_init (0: not offered) */
   uint32_t *ray_steps_out; /* device or NULL: tg_bpj_plan_count also writes every ray's Siddon
                               step count there, as tg_count_steps does (for the pj/pjt call
-                              with the same views, mask and obstacle) */
+                              with the same views, mask and obstacle); tg_bpj_plan_build then
+                              runs each ray for its count instead of testing every step
+                              against the map and the obstacle, so the counts must stay as
+                              the count pass left them until the build */
 } tg_bpj_plan;
 
 /* Fills the derived fields and meta_bytes for the geometry of *c (kind bpj or

@@ -126,21 +126,24 @@ def test_plan_count_step_counts_and_single_trace_fill(oracle, dtype, case):
             cls = lambda h, c, **k: Siddon4d(h, c, tindex=np.arange(6) % 3, **k)
     P1 = cls(hs, ch, dtype=dtype, bpj_mode="plan", **kw)
     P2 = cls(hs, ch, dtype=dtype, bpj_mode="plan", plan_single_trace=False, **kw)
+    # the fill that tests every step against the map and the obstacle again
+    P3 = cls(hs, ch, dtype=dtype, bpj_mode="plan", plan_counted_fill=False, **kw)
     P0 = cls(hs, ch, dtype=dtype, bpj_mode="atomic", **kw)
     steps_plan = P1.ray_steps().clone()  # written by the plan's count pass
     assert P1._plan_count is not None
     assert torch.equal(steps_plan, P0.ray_steps())
-    p1, p2 = P1.build_plan(), P2.build_plan()
-    assert p1.records == p2.records and p1.chunks == p2.chunks
+    p1, p2, p3 = P1.build_plan(), P2.build_plan(), P3.build_plan()
+    assert p1.records == p2.records == p3.records and p1.chunks == p2.chunks == p3.chunks
     rng = np.random.default_rng(4)
     y = torch.from_numpy(rng.uniform(size=P1.shapeout).astype(dtype)).cuda()
     b1 = P1.rmatvec(y).cpu().numpy()
-    b2 = P2.rmatvec(y).cpu().numpy()
-    if P1.acc64:  # float64 sums rounded once: equal but for one-ulp ties
-        d = b1 != b2
-        assert d.mean() <= 1e-3
-        assert np.all(np.abs(b1 - b2)[d] <= np.spacing(np.maximum(np.abs(b1), np.abs(b2)))[d])
-    else:  # float64 data: the brick sums' order follows the record order, which may differ
-        assert np.linalg.norm(b1 - b2) <= 1e-13 * max(np.linalg.norm(b2), 1e-300)
+    for P in (P2, P3):
+        b2 = P.rmatvec(y).cpu().numpy()
+        if P1.acc64:  # float64 sums rounded once: equal but for one-ulp ties
+            d = b1 != b2
+            assert d.mean() <= 1e-3
+            assert np.all(np.abs(b1 - b2)[d] <= np.spacing(np.maximum(np.abs(b1), np.abs(b2)))[d])
+        else:  # float64 data: the brick sums' order follows the record order, which may differ
+            assert np.linalg.norm(b1 - b2) <= 1e-13 * max(np.linalg.norm(b2), 1e-300)
     if case == "missing":
         assert p1.records == 0 and not b1.any()

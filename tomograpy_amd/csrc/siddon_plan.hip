@@ -170,6 +170,26 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
   const uint32_t rid = ray_id((uint32_t)a.n1, (uint32_t)a.n2, t, i, j);
   int last = -1;
   uint32_t seg = 0, total = 0;
+  if (a.ray_steps) {
+    // the ray's step count from the count pass, which walked this same
+    // traversal with the map and obstacle tests: the walk needs neither
+    const uint32_t n = a.ray_steps[rid];
+    for (uint32_t k = 0; k < n; k++) {
+      const int b = brick_of(g, r.iv, bin);
+      if (b != last) {
+        if (last >= 0) close(seg);
+        emit(b, r, rid, last < 0);
+        last = b;
+        seg = 0;
+      }
+      bool sx, sy, sz;
+      step(r, sx, sy, sz);
+      seg++;
+    }
+    if (last >= 0) close(seg);
+    if (out) *out = n;
+    return;
+  }
   int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
   while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
     if (--guard < 0) {
@@ -1164,7 +1184,9 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
       hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
                          counts, pl, per_ray, rm);
   });
-  p->pairs_ready = (pairs || rowmat) ? 1 : 0;
+  // bit 0: each trace workgroup's per-brick counts in scratch; bit 1: every
+  // ray's step count in ray_steps_out
+  p->pairs_ready = ((pairs || rowmat) ? 1 : 0) | (per_ray ? 2 : 0);
   if (rc) return rc;
   if ((rc = hip_check(hipGetLastError(), "plan count launch"))) return rc;
   std::vector<uint32_t> hc(g.nbricks);
@@ -1252,6 +1274,9 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   KArgs<T> a;
   int rc = make_args<T>(c, 0, c->nv, a, p->trace_tile > 0 ? p->trace_tile : kPlanTraceTile);
   if (rc) return rc;
+  // the count pass's per-ray step counts, when it wrote them: each ray then
+  // runs for its count without the map and obstacle tests
+  a.ray_steps = ((p->pairs_ready & 2) && p->ray_steps_out) ? p->ray_steps_out : nullptr;
   BrickGeom g;
   if ((rc = make_geom(c, p, g))) return rc;
   const MetaLayout m = meta_layout(g.nbricks);
@@ -1271,7 +1296,7 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   if (g.canon) {
     // ray order inside every brick, from the count pass's scanned count matrix
     const CanonScratch cs = canon_scratch((int64_t)c->nv * (int64_t)per_view, g.nbricks);
-    if (!p->pairs_ready || !p->scratch || p->scratch_bytes < cs.total)
+    if (!(p->pairs_ready & 1) || !p->scratch || p->scratch_bytes < cs.total)
       return set_error(TG_EINVAL, "a canonical plan's build needs the scratch its count filled");
     const uint32_t *rowmat = (const uint32_t *)((const char *)p->scratch + cs.mat);
     rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
@@ -1286,7 +1311,8 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
     if (rc) return rc;
     return hip_check(hipGetLastError(), "canonical plan fill launch");
   }
-  const uint32_t *pairs = (p->pairs_ready && p->scratch && p->scratch_bytes >= plan_scratch_bytes(a))
+  const uint32_t *pairs = ((p->pairs_ready & 1) && p->scratch &&
+                           p->scratch_bytes >= plan_scratch_bytes(a))
                               ? (const uint32_t *)p->scratch
                               : nullptr;
   rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {

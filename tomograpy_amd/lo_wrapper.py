@@ -84,14 +84,17 @@ class Siddon:
       image's second axis rounded up to a power of two, at most 256);
     * ``plan_single_trace`` (default on): the plan's count pass hands its
       per-workgroup record counts to the fill pass, which then traces every
-      ray once instead of twice.
+      ray once instead of twice;
+    * ``plan_counted_fill`` (default on): the fill pass runs every ray for the
+      step count the count pass wrote, without the per-step map and obstacle
+      tests.
     """
 
     def __init__(self, data_header, cube_header, mask=None, obstacle=None, dtype=None,
                  voxmask=None, pix_weight=None, vox_weight=None, deterministic=False,
                  tile_j=0, device=None, bpj_mode="auto", brick=None, chunk_records=0,
                  plan_order=2, plan_trace_tile=0, plan_tie_spread=None, acc64=None,
-                 counted_forward=True, plan_single_trace=True):
+                 counted_forward=True, plan_single_trace=True, plan_counted_fill=True):
         self.data_header = list(data_header)
         self.map_header = dict(cube_header)
         check_headers(self.data_header, self.map_header)
@@ -146,6 +149,7 @@ class Siddon:
         self._plan_count = None
         self._auto_plan = None
         self.plan_single_trace = bool(plan_single_trace)
+        self.plan_counted_fill = bool(plan_counted_fill)
         self.use_det_plan = True  # deterministic mode through a canonical plan when it fits
         self._det_ok = None
         self._det_plan = None
@@ -229,8 +233,8 @@ class Siddon:
                 self._ray_steps = torch.zeros(self.nv * self.n1 * self.n2, dtype=torch.int32,
                                               device=self.dev)
                 p.ray_steps_out = self._ray_steps.data_ptr()
+            # (the build reads the per-ray counts back from ray_steps_out)
             N.check(lib.tg_bpj_plan_count(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
-            p.ray_steps_out = None
             self._plan_count = (p, meta, scratch)
         return self._plan_count
 
@@ -256,12 +260,15 @@ class Siddon:
         rec = torch.empty(max(int(p.rec_bytes), 1), dtype=torch.uint8, device=self.dev)
         p.rec = rec.data_ptr()
         p.rec_bytes = rec.numel()
+        if not self.plan_counted_fill:
+            p.pairs_ready = int(p.pairs_ready) & ~2  # the fill tests every step again
         N.check(lib.tg_bpj_plan_build(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
         # the scratch is not needed once the fill has run; freeing it returns it to
         # torch's allocator, which orders its reuse after this stream's work
         p.scratch = None
         p.scratch_bytes = 0
         p.pairs_ready = 0
+        p.ray_steps_out = None
         del scratch
         self._plan = p
         self._plan_bufs = (meta, rec)

@@ -323,6 +323,10 @@ class Siddon:
         scratch = torch.empty(max(int(p.scratch_bytes), 1), dtype=torch.uint8, device=self.dev)
         p.scratch = scratch.data_ptr()
         p.scratch_bytes = scratch.numel()
+        # the count pass writes every ray's step count, which the canonical
+        # fill's two traces then run for without per-step map and obstacle tests
+        steps = torch.empty(self.nv * self.n1 * self.n2, dtype=torch.int32, device=self.dev)
+        p.ray_steps_out = steps.data_ptr()
         N.check(lib.tg_bpj_plan_count(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
         rec = torch.empty(max(int(p.rec_bytes), 1), dtype=torch.uint8, device=self.dev)
         p.rec = rec.data_ptr()
@@ -331,7 +335,8 @@ class Siddon:
         p.scratch = None
         p.scratch_bytes = 0
         p.pairs_ready = 0
-        del scratch
+        p.ray_steps_out = None
+        del scratch, steps
         self._det_plan = p
         self._det_plan_bufs = (meta, rec)
         return p

@@ -1608,14 +1608,17 @@ int run_det_plan(const tg_call *c, hipStream_t s) {
   if (tg_bpj_plan_init(&cb, &p) != TG_OK) return kDetPlanUnfit;
   char *ws = (char *)c->workspace;
   const size_t meta = align_up(p.meta_bytes, 256), scratch = align_up(p.scratch_bytes, 256);
-  if (!ws || c->workspace_bytes < meta + scratch + 256) return kDetPlanUnfit;
+  // every ray's step count, written by the count pass for the fill's traces
+  const size_t steps = align_up(4 * call_rays(c), 256);
+  if (!ws || c->workspace_bytes < meta + scratch + steps + 256) return kDetPlanUnfit;
   p.meta = ws;
   p.scratch = ws + meta;
+  p.ray_steps_out = (uint32_t *)(ws + meta + scratch);
   int rc = plan_count<T>(&cb, &p, s);
   if (rc) return rc;
   const size_t rec = rec_bytes<T>((size_t)p.records, call_rays(c));
-  if (c->workspace_bytes < meta + scratch + rec) return kDetPlanUnfit;
-  p.rec = ws + meta + scratch;
+  if (c->workspace_bytes < meta + scratch + steps + rec) return kDetPlanUnfit;
+  p.rec = ws + meta + scratch + steps;
   p.rec_bytes = rec;
   if ((rc = plan_build<T>(&cb, &p, s))) return rc;
   return plan_bpj<T>(c, &p, s);
@@ -1624,10 +1627,10 @@ int run_det_plan(const tg_call *c, hipStream_t s) {
 template int run_det_plan<float>(const tg_call *, hipStream_t);
 template int run_det_plan<double>(const tg_call *, hipStream_t);
 
-// Workspace bytes of run_det_plan for *c: meta, count matrix and an upper
-// bound of the records (every traced ray entering every brick on its way is
-// bounded by n0 / b0 + n1 / b1 + n2 / b2 + 3 bricks per ray); 0 when the
-// geometry does not fit a canonical plan.
+// Workspace bytes of run_det_plan for *c: meta, count matrix, per-ray step
+// counts and an upper bound of the records (every traced ray entering every
+// brick on its way is bounded by n0 / b0 + n1 / b1 + n2 / b2 + 3 bricks per
+// ray); 0 when the geometry does not fit a canonical plan.
 size_t det_plan_workspace_bytes(const tg_call *c) {
   tg_call cb = *c;
   cb.flags &= ~TG_DETERMINISTIC;
@@ -1640,7 +1643,8 @@ size_t det_plan_workspace_bytes(const tg_call *c) {
                                   c->map.n[2] / p.brick[2] + 3);
   const size_t rec = c->dtype == TG_F64 ? rec_bytes<double>(rays * per_ray, rays)
                                         : rec_bytes<float>(rays * per_ray, rays);
-  return align_up(p.meta_bytes, 256) + align_up(p.scratch_bytes, 256) + rec;
+  return align_up(p.meta_bytes, 256) + align_up(p.scratch_bytes, 256) +
+         align_up(4 * rays, 256) + rec;
 }
 
 }  // namespace tg

@@ -147,3 +147,29 @@ def test_plan_count_step_counts_and_single_trace_fill(oracle, dtype, case):
             assert np.linalg.norm(b1 - b2) <= 1e-13 * max(np.linalg.norm(b2), 1e-300)
     if case == "missing":
         assert p1.records == 0 and not b1.any()
+
+
+def test_plan_too_large_for_free_memory_falls_back(oracle):
+    """In "auto" mode an operator whose counted plan would not fit the free
+    device memory backprojects with the atomic scatter (forced here by a zero
+    share); the count pass's per-ray step counts still drive the forward."""
+    from tomograpy_amd import workloads as W
+    print("plan memory fallback", flush=True)
+    ch, cube, hs = W.config1(np.float32)
+    P = siddon_lo(hs, ch, dtype=np.float32)
+    P.PLAN_MAX_FREE_SHARE = 0.0
+    assert not P.use_plan()
+    assert P._ray_steps is not None and P._plan_count is None
+    x = torch.from_numpy(np.ascontiguousarray(cube)).cuda()
+    y_ref = np.zeros((128, 128, len(hs)), np.float32)
+    oracle.siddon(oracle.PJ, view_records(hs), map_record(ch), np.asarray(cube),
+                  T.InfoArray(y_ref, header=hs))
+    assert np.array_equal(P.matvec(x).cpu().numpy(), y_ref)
+    rng = np.random.default_rng(41)
+    y = rng.uniform(-1, 1, y_ref.shape).astype(np.float32)
+    ref = np.zeros(cube.shape, np.float32)
+    oracle.siddon(oracle.BPJ, view_records(hs), map_record(ch), ref, T.InfoArray(y, header=hs),
+                  nthread=1)
+    b = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
+    assert P._plan is None
+    assert np.linalg.norm(b - ref) <= 1e-5 * np.linalg.norm(ref)

@@ -202,7 +202,20 @@ class Siddon:
         if self._auto_plan is None:
             bound = self.n1 * self.n2 * self.nv * sum(self.cube_shape)
             self._auto_plan = bound >= self.PLAN_MIN_STEPS
+            if self._auto_plan:
+                # a plan that would take more than PLAN_MAX_FREE_SHARE of the
+                # device memory still free (torch's cached blocks included)
+                # leaves the backprojections to the atomic scatter
+                p, _, _ = self._count_plan()
+                free, _ = torch.cuda.mem_get_info(self.dev)
+                free += torch.cuda.memory_reserved(self.dev) - torch.cuda.memory_allocated(self.dev)
+                if int(p.rec_bytes) > self.PLAN_MAX_FREE_SHARE * free:
+                    self._auto_plan = False
+                    self._plan_count = None  # the per-ray step counts stay
         return self._auto_plan
+
+    # Share of the free device memory a plan may take in "auto" mode.
+    PLAN_MAX_FREE_SHARE = 0.8
 
     def _count_plan(self):
         """Counts the plan's records (synchronises); returns (plan, meta buffer)."""

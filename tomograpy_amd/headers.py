@@ -348,10 +348,17 @@ def _header_values(h, keys, what):
 
 def view_records(data_header):
     """(nv, 18) float64 array of the values the projector reads per view
-    (C_siddon.c.template:1077-1101); raises on missing or non-finite values."""
-    rec = np.empty((len(data_header), len(VIEW_KEYS)), dtype=np.float64)
-    for t, h in enumerate(data_header):
-        rec[t] = _header_values(h, VIEW_KEYS, "view %d" % t)
+    (C_siddon.c.template:1077-1101); raises on missing or non-finite values,
+    as _header_values does, naming the first such view and key."""
+    try:
+        rec = np.array([[float(h[key]) for key in VIEW_KEYS] for h in data_header],
+                       dtype=np.float64).reshape(len(data_header), len(VIEW_KEYS))
+    except KeyError:
+        rec = None
+    if rec is None or not np.isfinite(rec).all():
+        # the slow path names the offending view and key
+        for t, h in enumerate(data_header):
+            _header_values(h, VIEW_KEYS, "view %d" % t)
     return rec
 
 

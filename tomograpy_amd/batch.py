@@ -59,7 +59,7 @@ class _Ring:
     def release(self, b):
         """Marks the launch that read b.stage as enqueued on the current stream."""
         ev = torch.cuda.Event()
-        ev.record(torch.cuda.current_stream())
+        ev.record(S.current_stream())
         b.copied = ev
 
 

@@ -138,11 +138,16 @@ class SegTable:
             ctypes.memmove(ctypes.byref(segs[k]), ctypes.byref(g), ctypes.sizeof(N.TgCgSeg))
             block += int(lib.tg_cg_seg_blocks(s.n))
         self.nblocks = block
-        raw = ctypes.string_at(ctypes.addressof(segs), ctypes.sizeof(segs))
-        host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
-        # from pinned memory the upload is stream-ordered and does not wait for
-        # the work already queued
-        self.dev_table = host.pin_memory().to(dev, non_blocking=True)
+        # staged in pinned memory (a ring of staging buffers reused once their
+        # copies are done), so the upload is stream-ordered and does not wait
+        # for the work already queued
+        nbytes = ctypes.sizeof(segs)
+        ring = B._ring("cgseg", dev)
+        st = ring.take(nbytes, dev)
+        ctypes.memmove(st.stage.data_ptr(), ctypes.addressof(segs), nbytes)
+        self.dev_table = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.dev_table.copy_(st.stage[:nbytes], non_blocking=True)
+        ring.release(st)
         self.partials = torch.empty(max(block, 1), dtype=torch.float64, device=dev)
         self.code = solves[0].P.code
         self.launches = None  # the batched forward and backward of a step, packed once

@@ -205,13 +205,23 @@ class Siddon:
             if self._auto_plan:
                 # a plan that would take more than PLAN_MAX_FREE_SHARE of the
                 # device memory still free (torch's cached blocks included)
-                # leaves the backprojections to the atomic scatter
-                p, _, _ = self._count_plan()
+                # leaves the backprojections to the atomic scatter.  Most
+                # geometries are far from that: an upper bound of the records
+                # (each ray entering every brick layer on its way) is checked
+                # against the device's free memory first, and only a bound
+                # that does not fit counts the plan and asks torch's allocator
+                rays = self.n1 * self.n2 * self.nv
+                edge = min(self.brick) if self.brick is not None else 16  # smallest brick edge
+                layers = sum(-(-n // edge) for n in self.cube_shape) + 3
+                bound_bytes = rays * (layers + 1) * 32  # records and the ray table
                 free, _ = torch.cuda.mem_get_info(self.dev)
-                free += torch.cuda.memory_reserved(self.dev) - torch.cuda.memory_allocated(self.dev)
-                if int(p.rec_bytes) > self.PLAN_MAX_FREE_SHARE * free:
-                    self._auto_plan = False
-                    self._plan_count = None  # the per-ray step counts stay
+                if bound_bytes > self.PLAN_MAX_FREE_SHARE * free:
+                    p, _, _ = self._count_plan()
+                    free += (torch.cuda.memory_reserved(self.dev)
+                             - torch.cuda.memory_allocated(self.dev))
+                    if int(p.rec_bytes) > self.PLAN_MAX_FREE_SHARE * free:
+                        self._auto_plan = False
+                        self._plan_count = None  # the per-ray step counts stay
         return self._auto_plan
 
     # Share of the free device memory a plan may take in "auto" mode.
@@ -578,7 +588,7 @@ class Siddon4d(Siddon):
         self.nt = int(nt if nt is not None else tindex.max() + 1)
         if tindex.min() < 0 or tindex.max() >= self.nt:
             raise ValueError("time bins outside [0, %d)" % self.nt)
-        self.tindex = torch.from_numpy(tindex).to(self.dev)
+        self.tindex = S._upload(torch.from_numpy(tindex), self.dev)
         self.shapein = self.cube_shape + (self.nt,)
         self.xin = map4_buffer(*self.cube_shape, self.nt, self.tdtype, self.dev)
 

@@ -240,7 +240,7 @@ def _download(req, x):
         host = torch.empty(tuple(x.shape), dtype=x.dtype, pin_memory=req._pinned)
     host.copy_(x, non_blocking=host.is_pinned())
     ev = torch.cuda.Event()
-    ev.record()
+    ev.record(S.current_stream())
     return host, ev
 
 
@@ -327,7 +327,7 @@ class Server:
         self._scal_turn ^= 1
         host.copy_(self.scal_pool, non_blocking=True)
         ev = torch.cuda.Event()
-        ev.record()
+        ev.record(S.current_stream())
         return host, ev
 
     def run(self, requests, on_step=None, on_complete=None):

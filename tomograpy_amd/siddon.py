@@ -70,9 +70,11 @@ def check_status(dev=None, reset=True):
 
 
 def views_tensor(records, dtype_code, n1, n2, dev):
-    """Device buffer of packed view descriptors (cached by content)."""
+    """Device buffer of packed view descriptors (cached by content and stream)."""
     records = np.ascontiguousarray(records, dtype=np.float64)
-    key = (dtype_code, int(n1), int(n2), str(dev), records.shape, records.tobytes())
+    # (per stream: an entry's upload is ordered before its uses on that stream)
+    stream = current_stream().cuda_stream if torch.device(dev).type == "cuda" else 0
+    key = (dtype_code, int(n1), int(n2), str(dev), stream, records.shape, records.tobytes())
     hit = _VIEW_CACHE.get(key)
     if hit is not None:
         _VIEW_CACHE.move_to_end(key)
@@ -82,11 +84,19 @@ def views_tensor(records, dtype_code, n1, n2, dev):
     nbytes = lib.tg_views_bytes(dtype_code, nv, n1, n2)
     host = np.zeros(max(nbytes, 1), dtype=np.uint8)
     N.check(lib.tg_views_pack(dtype_code, nv, n1, n2, records.ctypes.data, host.ctypes.data, nbytes))
-    dev_buf = torch.from_numpy(host).to(dev)
+    dev_buf = _upload(torch.from_numpy(host), dev)
     _VIEW_CACHE[key] = dev_buf
     while len(_VIEW_CACHE) > _VIEW_CACHE_MAX:
         _VIEW_CACHE.popitem(last=False)
     return dev_buf
+
+
+def _upload(host, dev):
+    """Host tensor to ``dev``; to a GPU from pinned memory, so the copy does not
+    wait for the work already queued on the stream."""
+    if torch.device(dev).type != "cuda":
+        return host.to(dev)
+    return host.pin_memory().to(dev, non_blocking=True)
 
 
 def _ptr(t):
@@ -147,8 +157,9 @@ def make_call(kind, dtype_code, data_shape, data_strides, cube_shape, cube_strid
     return c
 
 
-# The stream handle fixed by fixed_stream(), else None.
+# The stream handle and torch stream fixed by fixed_stream(), else None.
 _FIXED_STREAM = None
+_FIXED_TORCH_STREAM = None
 
 
 def stream_handle():
@@ -158,19 +169,27 @@ def stream_handle():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def current_stream():
+    """torch's current stream (the one fixed by fixed_stream() inside it)."""
+    if _FIXED_TORCH_STREAM is not None:
+        return _FIXED_TORCH_STREAM
+    return torch.cuda.current_stream()
+
+
 @contextlib.contextmanager
 def fixed_stream():
-    """Inside the block, stream_handle() returns the stream current at entry
-    without asking torch again (about 9 us a call), for loops that make many
-    library calls on one stream.  The block must not switch streams, and other
-    threads must not call the library meanwhile."""
-    global _FIXED_STREAM
-    prev = _FIXED_STREAM
-    _FIXED_STREAM = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    """Inside the block, stream_handle() and current_stream() return the stream
+    current at entry without asking torch again (about 9 us a call), for loops
+    that make many library calls on one stream.  The block must not switch
+    streams, and other threads must not call the library meanwhile."""
+    global _FIXED_STREAM, _FIXED_TORCH_STREAM
+    prev, prev_t = _FIXED_STREAM, _FIXED_TORCH_STREAM
+    _FIXED_TORCH_STREAM = torch.cuda.current_stream()
+    _FIXED_STREAM = ctypes.c_void_p(_FIXED_TORCH_STREAM.cuda_stream)
     try:
         yield
     finally:
-        _FIXED_STREAM = prev
+        _FIXED_STREAM, _FIXED_TORCH_STREAM = prev, prev_t
 
 
 def det_workspace(call, dev, budget_bytes=None):
@@ -219,7 +238,7 @@ def _mask_tensor(mask, shape, dev):
     if isinstance(mask, torch.Tensor):
         m = mask.to(dev)
     else:
-        m = torch.from_numpy(np.ascontiguousarray(np.asarray(mask))).to(dev)
+        m = _upload(torch.from_numpy(np.ascontiguousarray(np.asarray(mask))), dev)
     if tuple(m.shape) != tuple(shape):
         raise ValueError("mask shape %s does not match data shape %s" % (tuple(m.shape), tuple(shape)))
     # Skip a pixel iff its mask value is nonzero (C_siddon.c.template:520).

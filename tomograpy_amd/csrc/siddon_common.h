@@ -280,6 +280,26 @@ __device__ __forceinline__ uint32_t wave_floor(uint32_t x, uint32_t c) {
   }
 }
 
+// step() with the minimum taken by one min3, as step_walk takes it: the same
+// axes, distances and indices for the finite distances of a traversal (the
+// value of the minimum is the length the cascade picks, up to the sign of a
+// zero, and no distance is -0).
+template <typename T>
+__device__ __forceinline__ T step_min(Ray<T> &r, bool &sx, bool &sy, bool &sz) {
+  const T d = tmin(tmin(r.D[0], r.D[1]), r.D[2]);
+  sx = (r.D[0] == d);
+  sy = (r.D[1] == d);
+  sz = (r.D[2] == d);
+  r.ac += d;
+  r.D[0] = sx ? r.pabs[0] : r.D[0] - d;
+  r.D[1] = sy ? r.pabs[1] : r.D[1] - d;
+  r.D[2] = sz ? r.pabs[2] : r.D[2] - d;
+  r.iv[0] += sx ? r.up[0] : 0;
+  r.iv[1] += sy ? r.up[1] : 0;
+  r.iv[2] += sz ? r.up[2] : 0;
+  return d;
+}
+
 template <bool FAST>
 struct Walk;
 

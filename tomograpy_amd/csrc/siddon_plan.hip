@@ -62,6 +62,10 @@ struct BrickGeom {
   int32_t spread;     // low bits of the pass position reversed to order ties
   int32_t canon;      // order 3: each brick's records in ray order (view, row, column),
                       // cells unpadded, for the deterministic backprojection
+  int32_t ks[2];      // brick key shifts of axes 0 and 1 (see brick_key)
+  int32_t nkeys;      // brick keys per time bin: nb0 << ks0
+  int32_t keyed;      // nkeys * nt <= kHistMax: per-workgroup counts in LDS, indexed by key
+  int32_t direct;     // fill: LDS cursors hold record positions (< 2^32, no shuffle)
 };
 
 // A record's ray id is the ray's linear index (t n1 + i) n2 + j: the index of
@@ -131,13 +135,37 @@ __device__ __forceinline__ uint32_t brick_local(const BrickGeom &g, const int iv
                     (iv[1] & (g.b[1] - 1)) * brick_pitch1(g) + (iv[2] & (g.b[2] - 1)));
 }
 
-__device__ __forceinline__ int brick_of(const BrickGeom &g, const int iv[3], int bin) {
-  return ((((iv[0] >> g.lg[0]) * g.nb[1]) + (iv[1] >> g.lg[1])) * g.nb[2] + (iv[2] >> g.lg[2])) +
-         bin * g.nb[0] * g.nb[1] * g.nb[2];
+// A brick's key: its coordinates packed as (bx << ks0) | (by << ks1) | bz,
+// ks1 = ceil(log2 nb2) and ks0 = ks1 + ceil(log2 nb1), plus bin x nkeys for
+// time bin `bin`.  Shifts and ors only, so a trace forms the key of its voxel
+// at every step, and the per-workgroup counts in LDS are indexed by it: keys
+// number the bricks of a bin with gaps where nb1 or nb2 is not a power of two.
+__device__ __forceinline__ uint32_t brick_key(const BrickGeom &g, const int iv[3]) {
+  return ((uint32_t)(iv[0] >> g.lg[0]) << g.ks[0]) | ((uint32_t)(iv[1] >> g.lg[1]) << g.ks[1]) |
+         (uint32_t)(iv[2] >> g.lg[2]);
+}
+constexpr uint32_t kNoBrick = 0xffffffffu;  // no key: keys stay under 2^31
+
+// The brick index (axis 2 fastest, then axes 1 and 0, then bins) of a key of
+// bin `bin`, and the key of a brick index.
+__host__ __device__ __forceinline__ int brick_of_key(const BrickGeom &g, uint32_t key, int bin) {
+  const uint32_t k = key - (uint32_t)bin * (uint32_t)g.nkeys;
+  const int bx = (int)(k >> g.ks[0]);
+  const int by = (int)((k >> g.ks[1]) & ((1u << (g.ks[0] - g.ks[1])) - 1u));
+  const int bz = (int)(k & ((1u << g.ks[1]) - 1u));
+  return ((bx * g.nb[1] + by) * g.nb[2] + bz) + bin * (g.nb[0] * g.nb[1] * g.nb[2]);
+}
+__host__ __device__ __forceinline__ uint32_t key_of_brick(const BrickGeom &g, int b) {
+  const int nsp = g.nb[0] * g.nb[1] * g.nb[2];
+  const int bin = b / nsp, r = b - bin * nsp;
+  const int bz = r % g.nb[2], by = (r / g.nb[2]) % g.nb[1], bx = r / (g.nb[2] * g.nb[1]);
+  return (((uint32_t)bx << g.ks[0]) | ((uint32_t)by << g.ks[1]) | (uint32_t)bz) +
+         (uint32_t)bin * (uint32_t)g.nkeys;
 }
 
-// Traces the ray of (tile, lane); calls emit(brick, ray, id, first) at the
-// first step in every brick and close(steps) when the ray leaves that brick
+// Traces the ray of (tile, lane); calls emit(key, bin, ray, id, first) at the
+// first step in every brick (key: brick_key + bin x nkeys) and close(steps)
+// when the ray leaves that brick
 // (or the map, or meets the Sun) with the number of steps taken in it.  Same
 // traversal as the projector kernels.  With per_ray (not null) it also writes
 // the ray's total step count there, indexed (t * n1 + i) * n2 + j as
@@ -168,25 +196,26 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
     }
   }
   const uint32_t rid = ray_id((uint32_t)a.n1, (uint32_t)a.n2, t, i, j);
-  int last = -1;
+  const uint32_t kboff = (uint32_t)bin * (uint32_t)g.nkeys;
+  uint32_t last = kNoBrick;
   uint32_t seg = 0, total = 0;
   if (a.ray_steps) {
     // the ray's step count from the count pass, which walked this same
     // traversal with the map and obstacle tests: the walk needs neither
     const uint32_t n = a.ray_steps[rid];
     for (uint32_t k = 0; k < n; k++) {
-      const int b = brick_of(g, r.iv, bin);
-      if (b != last) {
-        if (last >= 0) close(seg);
-        emit(b, r, rid, last < 0);
-        last = b;
+      const uint32_t key = brick_key(g, r.iv);
+      if (key != last) {
+        if (last != kNoBrick) close(seg);
+        emit(key + kboff, bin, r, rid, last == kNoBrick);
+        last = key;
         seg = 0;
       }
       bool sx, sy, sz;
-      step(r, sx, sy, sz);
+      step_min(r, sx, sy, sz);
       seg++;
     }
-    if (last >= 0) close(seg);
+    if (last != kNoBrick) close(seg);
     if (out) *out = n;
     return;
   }
@@ -196,19 +225,19 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
       flag_status(a.status, TG_EGEOMETRY);
       break;
     }
-    const int b = brick_of(g, r.iv, bin);
-    if (b != last) {
-      if (last >= 0) close(seg);
-      emit(b, r, rid, last < 0);
-      last = b;
+    const uint32_t key = brick_key(g, r.iv);
+    if (key != last) {
+      if (last != kNoBrick) close(seg);
+      emit(key + kboff, bin, r, rid, last == kNoBrick);
+      last = key;
       seg = 0;
     }
     bool sx, sy, sz;
-    step(r, sx, sy, sz);
+    step_min(r, sx, sy, sz);
     seg++;
     total++;
   }
-  if (last >= 0) close(seg);
+  if (last != kNoBrick) close(seg);
   if (out) *out = total;
 }
 
@@ -230,9 +259,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
                                                            uint32_t *per_ray, uint32_t *rowmat) {
   __shared__ uint32_t hist[kHistMax];
   __shared__ uint32_t npairs;
-  const bool lds = g.nbricks <= kHistMax;
+  const bool lds = g.keyed;
+  const int nkeys = g.nkeys * g.nt;
   if (lds) {
-    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) hist[k] = 0;
+    for (int k = threadIdx.x; k < nkeys; k += blockDim.x) hist[k] = 0;
     if (threadIdx.x == 0) npairs = 0;
     __syncthreads();
   }
@@ -240,21 +270,22 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
   if (pairs) pairs += (size_t)blockIdx.x * (kWgPairBytes / 4);
   trace_bricks<T, SUN>(
       a, g, tile,
-      [&](int b, const Ray<T> &, uint32_t, bool) {
-        if (lds) atomicAdd(&hist[b], 1u);
-        else atomicAdd(&counts[b], 1u);
+      [&](uint32_t key, int bin, const Ray<T> &, uint32_t, bool) {
+        if (lds) atomicAdd(&hist[key], 1u);
+        else atomicAdd(&counts[brick_of_key(g, key, bin)], 1u);
       },
       [](uint32_t) {}, per_ray);
   if (lds) {
     __syncthreads();
-    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) {
+    for (int k = threadIdx.x; k < nkeys; k += blockDim.x) {
       const uint32_t h = hist[k];
       if (!h) continue;
-      atomicAdd(&counts[k], h);
+      const int b = brick_of_key(g, (uint32_t)k, g.nt > 1 ? k / g.nkeys : 0);
+      atomicAdd(&counts[b], h);
       if (pairs) {
         const uint32_t q = atomicAdd(&npairs, 1u);
         if (q < (uint32_t)kWgPairs) {
-          pairs[2 + 2 * q] = (uint32_t)k;
+          pairs[2 + 2 * q] = (uint32_t)b;
           pairs[3 + 2 * q] = h;
         }
       }
@@ -263,7 +294,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
     if (pairs && threadIdx.x == 0) pairs[0] = npairs <= (uint32_t)kWgPairs ? npairs : kPairsOverflow;
     if (rowmat) {
       uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
-      for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) row[k] = hist[k];
+      for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) row[k] = hist[key_of_brick(g, k)];
     }
   } else if (pairs && threadIdx.x == 0) {
     pairs[0] = kPairsOverflow;
@@ -341,13 +372,16 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, Bric
   const uint32_t bit = 1u << (lane & 31);
   trace_bricks<T, SUN>(
       a, g, tile,
-      [&](int b, const Ray<T> &, uint32_t, bool) { atomicOr(&mask[b * 8 + (lane >> 5)], bit); },
+      [&](uint32_t key, int bin, const Ray<T> &, uint32_t, bool) {
+        atomicOr(&mask[brick_of_key(g, key, bin) * 8 + (lane >> 5)], bit);
+      },
       [](uint32_t) {});
   __syncthreads();
   const uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
   unsigned long long pcur = 0;
   uint32_t lcur = 0;
-  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &r, uint32_t rid, bool first) {
+  trace_bricks<T, SUN>(a, g, tile, [&](uint32_t key, int bin, const Ray<T> &r, uint32_t rid,
+                                       bool first) {
     if (first) {
       RayEntry<T> e;
       e.pabs[0] = r.pabs[0];
@@ -356,6 +390,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, Bric
       e.up = pack_up(r.up);
       rec.ray[rid] = e;
     }
+    const int b = brick_of_key(g, key, bin);
     const uint32_t *m = &mask[b * 8];
     uint32_t rank = 0;
     for (uint32_t w = 0; w < (lane >> 5); w++) rank += __popc(m[w]);
@@ -373,38 +408,51 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, Bric
 // pairs: this workgroup's slot of the count pass's pairs (or null): when it
 // holds the workgroup's per-brick record counts, the ranges are reserved from
 // them and the rays are traced once; otherwise they are counted again here.
+// With g.direct the LDS cursor of a brick key holds the position of the
+// workgroup's next record in the plan (its range's start + the records
+// written), so a record is placed by one LDS atomic, without the brick index
+// and its offset.
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom g,
                                                           const unsigned long long *offsets,
                                                           uint32_t *cursor, const uint32_t *mult,
                                                           RecPtrs<T> rec, const uint32_t *pairs) {
   __shared__ uint32_t slot[kHistMax];
-  const bool lds = g.nbricks <= kHistMax;
+  const bool lds = g.keyed;
+  const bool direct = lds && g.direct;
+  const int nkeys = g.nkeys * g.nt;
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   if (pairs) pairs += (size_t)blockIdx.x * (kWgPairBytes / 4);
   if (lds) {
-    for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) slot[k] = 0;
+    for (int k = threadIdx.x; k < nkeys; k += blockDim.x) slot[k] = 0;
     __syncthreads();
     const uint32_t np = pairs ? pairs[0] : kPairsOverflow;
     if (np != kPairsOverflow) {
       // reserve one contiguous range per brick for this workgroup's records
       for (uint32_t q = threadIdx.x; q < np; q += blockDim.x) {
         const uint32_t b = pairs[2 + 2 * q];
-        slot[b] = atomicAdd(&cursor[b], pairs[3 + 2 * q]);
+        const uint32_t start = atomicAdd(&cursor[b], pairs[3 + 2 * q]);
+        slot[key_of_brick(g, (int)b)] = direct ? (uint32_t)(offsets[b] + start) : start;
       }
     } else {
       trace_bricks<T, SUN>(
-          a, g, tile, [&](int b, const Ray<T> &, uint32_t, bool) { atomicAdd(&slot[b], 1u); },
+          a, g, tile,
+          [&](uint32_t key, int, const Ray<T> &, uint32_t, bool) { atomicAdd(&slot[key], 1u); },
           [](uint32_t) {});
       __syncthreads();
-      for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x)
-        if (slot[k]) slot[k] = atomicAdd(&cursor[k], slot[k]);
+      for (int k = threadIdx.x; k < nkeys; k += blockDim.x) {
+        if (!slot[k]) continue;
+        const int b = brick_of_key(g, (uint32_t)k, g.nt > 1 ? k / g.nkeys : 0);
+        const uint32_t start = atomicAdd(&cursor[b], slot[k]);
+        slot[k] = direct ? (uint32_t)(offsets[b] + start) : start;
+      }
     }
     __syncthreads();
   }
   unsigned long long pcur = 0;
   uint32_t lcur = 0;
-  trace_bricks<T, SUN>(a, g, tile, [&](int b, const Ray<T> &r, uint32_t rid, bool first) {
+  trace_bricks<T, SUN>(a, g, tile, [&](uint32_t key, int bin, const Ray<T> &r, uint32_t rid,
+                                       bool first) {
     if (first) {
       RayEntry<T> e;
       e.pabs[0] = r.pabs[0];
@@ -413,13 +461,19 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
       e.up = pack_up(r.up);
       rec.ray[rid] = e;
     }
-    const uint32_t s = lds ? atomicAdd(&slot[b], 1u) : atomicAdd(&cursor[b], 1u);
-    unsigned long long q = s;
-    if (g.shuffle) {  // a bijection of [0, count): mult[b] is coprime with the count
-      const unsigned long long cnt = offsets[b + 1] - offsets[b];
-      q = (q * mult[b]) % cnt;
+    unsigned long long p;
+    if (direct) {
+      p = atomicAdd(&slot[key], 1u);
+    } else {
+      const int b = brick_of_key(g, key, bin);
+      const uint32_t s = lds ? atomicAdd(&slot[key], 1u) : atomicAdd(&cursor[b], 1u);
+      unsigned long long q = s;
+      if (g.shuffle) {  // a bijection of [0, count): mult[b] is coprime with the count
+        const unsigned long long cnt = offsets[b + 1] - offsets[b];
+        q = (q * mult[b]) % cnt;
+      }
+      p = offsets[b] + q;
     }
-    const unsigned long long p = offsets[b] + q;
     rec.rid[p] = rid;
     rec.d0[p] = r.D[0];
     rec.d1[p] = r.D[1];
@@ -1013,6 +1067,7 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
 }
 
+// ceil(log2 x): the exponent of the smallest power of two >= x
 int ilog2(int x) {
   int l = 0;
   while ((1 << l) < x) l++;
@@ -1040,6 +1095,13 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
   const long long nb = (long long)g.nb[0] * g.nb[1] * g.nb[2] * g.nt;
   if (nb > (1 << 30)) return set_error(TG_EINVAL, "too many bricks");
   g.nbricks = (int)nb;
+  g.ks[1] = ilog2(g.nb[2]);
+  g.ks[0] = g.ks[1] + ilog2(g.nb[1]);
+  const long long keys = ((long long)g.nb[0] << g.ks[0]) * g.nt;
+  if (keys >= (1ll << 31)) return set_error(TG_EINVAL, "too many bricks");
+  g.nkeys = g.nb[0] << g.ks[0];
+  g.keyed = keys <= kHistMax ? 1 : 0;
+  g.direct = 0;
   g.chunk = p->chunk_records > 0 ? p->chunk_records : 32768;
   g.shuffle = p->order == 1 ? 1 : 0;
   g.sort = p->order == 2 ? 1 : 0;
@@ -1054,6 +1116,10 @@ int make_geom(const tg_call *c, const tg_bpj_plan *p, BrickGeom &g) {
     if (g.nbricks > kDetMaxBricks)
       return set_error(TG_EINVAL, "a canonical plan holds at most %d bricks x time bins, not %d",
                        kDetMaxBricks, g.nbricks);
+    // the count pass writes each tile's row of counts from its LDS histogram
+    if (!g.keyed)
+      return set_error(TG_EINVAL, "a canonical plan holds at most %d brick keys x time bins",
+                       kHistMax);
     if (p->trace_tile != 0 && p->trace_tile != kPlanTraceTile)
       return set_error(TG_EINVAL, "a canonical plan traces rows of %d pixels", kPlanTraceTile);
     if (g.b[0] * g.b[1] * g.b[2] > 8192)
@@ -1279,6 +1345,8 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   a.ray_steps = ((p->pairs_ready & 2) && p->ray_steps_out) ? p->ray_steps_out : nullptr;
   BrickGeom g;
   if ((rc = make_geom(c, p, g))) return rc;
+  // record positions fit the fill's 32-bit LDS cursors
+  g.direct = (!g.shuffle && (unsigned long long)p->records < (1ull << 32)) ? 1 : 0;
   const MetaLayout m = meta_layout(g.nbricks);
   if (!p->meta || p->meta_bytes < m.total) return set_error(TG_EINVAL, "plan meta buffer too small");
   if (!p->rec || p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
@@ -1535,7 +1603,7 @@ int tg_bpj_plan_init(const tg_call *c, tg_bpj_plan *p) {
     const size_t per_tile = plan_scratch_bytes_for(c, p) / kWgPairBytes;
     p->scratch_bytes = canon_scratch((int64_t)per_tile, g.nbricks).total;
   } else {
-    p->scratch_bytes = g.nbricks <= kHistMax ? plan_scratch_bytes_for(c, p) : 0;
+    p->scratch_bytes = g.keyed ? plan_scratch_bytes_for(c, p) : 0;
   }
   p->pairs_ready = 0;
   p->records = 0;

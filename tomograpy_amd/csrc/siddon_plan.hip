@@ -251,12 +251,10 @@ constexpr size_t kWgPairBytes = 8 * (size_t)(kWgPairs + 1);
 
 // pairs: the workgroup's kWgPairBytes slot of per-workgroup pairs (or null);
 // per_ray: per-ray step counts out (or null).
-// rowmat (canonical plans, or null): this launch's rows of the tile x brick
-// matrix of record counts; the workgroup writes its tile's row.
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom g,
                                                            uint32_t *counts, uint32_t *pairs,
-                                                           uint32_t *per_ray, uint32_t *rowmat) {
+                                                           uint32_t *per_ray) {
   __shared__ uint32_t hist[kHistMax];
   __shared__ uint32_t npairs;
   const bool lds = g.keyed;
@@ -292,10 +290,6 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
     }
     __syncthreads();
     if (pairs && threadIdx.x == 0) pairs[0] = npairs <= (uint32_t)kWgPairs ? npairs : kPairsOverflow;
-    if (rowmat) {
-      uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
-      for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) row[k] = hist[key_of_brick(g, k)];
-    }
   } else if (pairs && threadIdx.x == 0) {
     pairs[0] = kPairsOverflow;
   }
@@ -313,6 +307,62 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
 constexpr int kDetMaxBricks = 2048;  // 256-bit lane masks of all bricks in LDS (64 KiB)
 constexpr int kDetSegRows = 256;     // rows per segment of the column scan
 constexpr int kDetMinSlots = 256;    // fewest contribution slots of one deterministic batch
+// The count pass of a canonical plan builds every tile's lane masks as it
+// counts, and hands them to the fill in a list per tile: a header word (the
+// number of bricks the tile's rays enter, or kPairsOverflow), then up to
+// kCanonList brick indices and their 8 mask words each.  A tile of C2's
+// geometry enters up to about 180 bricks of 32 x 16 x 16.
+constexpr int kCanonList = 255;
+constexpr size_t kCanonListWords = 2304;  // 1 + 9 * kCanonList, rounded up to 256 bytes
+static_assert(kCanonListWords >= 1 + 9 * (size_t)kCanonList, "canonical mask list too short");
+
+// The canonical count: per-brick lane masks in LDS (g.nbricks x 8 words of
+// dynamic LDS), then the tile's row of the count matrix (popcounts), the
+// global per-brick counts and the tile's mask list.
+template <typename T, bool SUN>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_count_canon(KArgs<T> a, BrickGeom g,
+                                                                 uint32_t *counts,
+                                                                 uint32_t *per_ray,
+                                                                 uint32_t *rowmat,
+                                                                 uint32_t *lists) {
+  extern __shared__ uint32_t cmask[];
+  __shared__ uint32_t nlist;
+  const int words = g.nbricks * (kPlanBlock / 32);
+  for (int k = threadIdx.x; k < words; k += blockDim.x) cmask[k] = 0;
+  if (threadIdx.x == 0) nlist = 0;
+  __syncthreads();
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t lane = threadIdx.x;
+  const uint32_t bit = 1u << (lane & 31);
+  trace_bricks<T, SUN>(
+      a, g, tile,
+      [&](uint32_t key, int bin, const Ray<T> &, uint32_t, bool) {
+        atomicOr(&cmask[brick_of_key(g, key, bin) * 8 + (lane >> 5)], bit);
+      },
+      [](uint32_t) {}, per_ray);
+  __syncthreads();
+  uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
+  uint32_t *list = lists + (size_t)tile * kCanonListWords;
+  for (int b = threadIdx.x; b < g.nbricks; b += blockDim.x) {
+    uint32_t m[8], cnt = 0;
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      m[w] = cmask[b * 8 + w];
+      cnt += __popc(m[w]);
+    }
+    row[b] = cnt;
+    if (!cnt) continue;
+    atomicAdd(&counts[b], cnt);
+    const uint32_t q = atomicAdd(&nlist, 1u);
+    if (q < (uint32_t)kCanonList) {
+      list[1 + q] = (uint32_t)b;
+#pragma unroll
+      for (int w = 0; w < 8; w++) list[1 + kCanonList + 8 * q + w] = m[w];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) list[0] = nlist <= (uint32_t)kCanonList ? nlist : kPairsOverflow;
+}
 
 // Segment sums of the columns: block (s, cb) sums rows [s * 256, (s + 1) * 256)
 // of columns [cb * 256, cb * 256 + 256) into seg[s * nb + col].
@@ -355,29 +405,49 @@ __global__ __launch_bounds__(256) void k_det_colapply(uint32_t *__restrict__ mat
   }
 }
 
-// Fill of a canonical plan: two traces per ray, the first to set each
-// brick's lane mask, the second to write each record at the brick's offset +
-// the tile's scanned count + the number of earlier lanes in the mask.
+// Fill of a canonical plan: each brick's lane mask (from the count pass's list
+// for the tile, or, when the list overflowed, from a first trace), then a
+// trace that writes each record at the brick's offset + the tile's scanned
+// count + the number of earlier lanes in the mask.  With g.direct the first
+// two are summed per brick into LDS before the trace (record positions are
+// under 2^32), so a record's position takes no global load.
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, BrickGeom g,
                                                                 const unsigned long long *offsets,
                                                                 const uint32_t *rowmat,
+                                                                const uint32_t *lists,
                                                                 RecPtrs<T> rec) {
-  __shared__ uint32_t mask[kDetMaxBricks * (kPlanBlock / 32)];
+  __shared__ __align__(16) uint32_t mask[kDetMaxBricks * (kPlanBlock / 32)];
+  __shared__ uint32_t base[kDetMaxBricks];
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const int words = g.nbricks * (kPlanBlock / 32);
   for (int k = threadIdx.x; k < words; k += blockDim.x) mask[k] = 0;
   __syncthreads();
   const uint32_t lane = threadIdx.x;
   const uint32_t bit = 1u << (lane & 31);
-  trace_bricks<T, SUN>(
-      a, g, tile,
-      [&](uint32_t key, int bin, const Ray<T> &, uint32_t, bool) {
-        atomicOr(&mask[brick_of_key(g, key, bin) * 8 + (lane >> 5)], bit);
-      },
-      [](uint32_t) {});
-  __syncthreads();
   const uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
+  const uint32_t *list = lists + (size_t)tile * kCanonListWords;
+  const uint32_t nl = list[0];
+  if (nl != kPairsOverflow) {
+    for (uint32_t e = threadIdx.x; e < 8 * nl; e += blockDim.x)
+      mask[list[1 + (e >> 3)] * 8 + (e & 7)] = list[1 + kCanonList + e];
+    if (g.direct)
+      for (uint32_t q = threadIdx.x; q < nl; q += blockDim.x) {
+        const uint32_t b = list[1 + q];
+        base[b] = (uint32_t)(offsets[b] + row[b]);
+      }
+  } else {
+    trace_bricks<T, SUN>(
+        a, g, tile,
+        [&](uint32_t key, int bin, const Ray<T> &, uint32_t, bool) {
+          atomicOr(&mask[brick_of_key(g, key, bin) * 8 + (lane >> 5)], bit);
+        },
+        [](uint32_t) {});
+    if (g.direct)
+      for (int b = threadIdx.x; b < g.nbricks; b += blockDim.x)
+        base[b] = (uint32_t)(offsets[b] + row[b]);
+  }
+  __syncthreads();
   unsigned long long pcur = 0;
   uint32_t lcur = 0;
   trace_bricks<T, SUN>(a, g, tile, [&](uint32_t key, int bin, const Ray<T> &r, uint32_t rid,
@@ -391,11 +461,16 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, Bric
       rec.ray[rid] = e;
     }
     const int b = brick_of_key(g, key, bin);
-    const uint32_t *m = &mask[b * 8];
+    // the lanes before this one that enter brick b: whole 64-bit mask words
+    // of the earlier wavefronts (a count uniform per wavefront), then this
+    // wavefront's word below the lane
+    const unsigned long long *m = reinterpret_cast<const unsigned long long *>(&mask[b * 8]);
+    const uint32_t wv = lane >> 6;
     uint32_t rank = 0;
-    for (uint32_t w = 0; w < (lane >> 5); w++) rank += __popc(m[w]);
-    rank += __popc(m[lane >> 5] & (bit - 1u));
-    const unsigned long long p = offsets[b] + row[b] + rank;
+    for (uint32_t w = 0; w < wv; w++) rank += (uint32_t)__popcll(m[w]);
+    rank += (uint32_t)__popcll(m[wv] & ((1ull << (lane & 63)) - 1ull));
+    const unsigned long long p =
+        g.direct ? (unsigned long long)(base[b] + rank) : offsets[b] + row[b] + rank;
     rec.rid[p] = rid;
     rec.d0[p] = r.D[0];
     rec.d1[p] = r.D[1];
@@ -1156,8 +1231,10 @@ MetaLayout meta_layout(int nbricks) {
 
 // Scratch of a canonical plan: the tile x brick count matrix and the segment
 // sums of its column scan.
+// Scratch of a canonical plan's build: the count matrix (tiles x bricks),
+// its column segment sums and every tile's mask list.
 struct CanonScratch {
-  size_t mat, seg, total;
+  size_t mat, seg, lists, total;
   int64_t rows, nseg;
 };
 
@@ -1167,7 +1244,8 @@ CanonScratch canon_scratch(int64_t tiles, int nbricks) {
   c.nseg = (tiles + kDetSegRows - 1) / kDetSegRows;
   c.mat = 0;
   c.seg = align_up(4 * (size_t)tiles * (size_t)nbricks, 256);
-  c.total = align_up(c.seg + 4 * (size_t)c.nseg * (size_t)nbricks, 256);
+  c.lists = align_up(c.seg + 4 * (size_t)c.nseg * (size_t)nbricks, 256);
+  c.total = align_up(c.lists + 4 * kCanonListWords * (size_t)tiles, 256);
   return c;
 }
 
@@ -1239,19 +1317,32 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
                         ? (uint32_t *)p->scratch
                         : nullptr;
   uint32_t *rowmat = g.canon ? (uint32_t *)((char *)p->scratch + cs.mat) : nullptr;
+  uint32_t *lists = g.canon ? (uint32_t *)((char *)p->scratch + cs.lists) : nullptr;
   uint32_t *per_ray = p->ray_steps_out;
   rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
     uint32_t *pl = pairs ? pairs + (size_t)av.t0 * per_view * (kWgPairBytes / 4) : nullptr;
-    uint32_t *rm = rowmat ? rowmat + (size_t)av.t0 * per_view * (size_t)g.nbricks : nullptr;
+    if (g.canon) {
+      uint32_t *rm = rowmat + (size_t)av.t0 * per_view * (size_t)g.nbricks;
+      uint32_t *ls = lists + (size_t)av.t0 * per_view * kCanonListWords;
+      const size_t lds = (size_t)g.nbricks * (kPlanBlock / 32) * 4;
+      if (sun)
+        hipLaunchKernelGGL((k_plan_count_canon<T, true>), dim3(blocks), dim3(kPlanBlock), lds, s,
+                           av, g, counts, per_ray, rm, ls);
+      else
+        hipLaunchKernelGGL((k_plan_count_canon<T, false>), dim3(blocks), dim3(kPlanBlock), lds, s,
+                           av, g, counts, per_ray, rm, ls);
+      return;
+    }
     if (sun)
       hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g, counts,
-                         pl, per_ray, rm);
+                         pl, per_ray);
     else
       hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
-                         counts, pl, per_ray, rm);
+                         counts, pl, per_ray);
   });
-  // bit 0: each trace workgroup's per-brick counts in scratch; bit 1: every
-  // ray's step count in ray_steps_out
+  // bit 0: each trace workgroup's per-brick counts (a canonical plan: the
+  // count matrix and the tiles' mask lists) in scratch; bit 1: every ray's
+  // step count in ray_steps_out
   p->pairs_ready = ((pairs || rowmat) ? 1 : 0) | (per_ray ? 2 : 0);
   if (rc) return rc;
   if ((rc = hip_check(hipGetLastError(), "plan count launch"))) return rc;
@@ -1367,14 +1458,16 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
     if (!(p->pairs_ready & 1) || !p->scratch || p->scratch_bytes < cs.total)
       return set_error(TG_EINVAL, "a canonical plan's build needs the scratch its count filled");
     const uint32_t *rowmat = (const uint32_t *)((const char *)p->scratch + cs.mat);
+    const uint32_t *lists = (const uint32_t *)((const char *)p->scratch + cs.lists);
     rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
       const uint32_t *rm = rowmat + (size_t)av.t0 * per_view * (size_t)g.nbricks;
+      const uint32_t *ls = lists + (size_t)av.t0 * per_view * kCanonListWords;
       if (sun)
         hipLaunchKernelGGL((k_plan_fill_canon<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av,
-                           g, offsets, rm, rp);
+                           g, offsets, rm, ls, rp);
       else
         hipLaunchKernelGGL((k_plan_fill_canon<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av,
-                           g, offsets, rm, rp);
+                           g, offsets, rm, ls, rp);
     });
     if (rc) return rc;
     return hip_check(hipGetLastError(), "canonical plan fill launch");

@@ -65,7 +65,8 @@ struct BrickGeom {
   int32_t ks[2];      // brick key shifts of axes 0 and 1 (see brick_key)
   int32_t nkeys;      // brick keys per time bin: nb0 << ks0
   int32_t keyed;      // nkeys * nt <= kHistMax: per-workgroup counts in LDS, indexed by key
-  int32_t direct;     // fill: LDS cursors hold record positions (< 2^32, no shuffle)
+  int32_t direct;     // fill: LDS cursors hold record positions (records x max(4, sizeof(T))
+                      // bytes < 2^32, no shuffle)
 };
 
 // A record's ray id is the ray's linear index (t n1 + i) n2 + j: the index of
@@ -129,10 +130,18 @@ __host__ __device__ __forceinline__ long long brick_lds_cells(const BrickGeom &g
 }
 
 // The LDS cell of voxel iv inside its brick: what a record stores in the low
-// 16 bits of lq.
+// 16 bits of lq.  (24-bit multiplies: a brick's cells fit the LDS, under 2^15.)
 __device__ __forceinline__ uint32_t brick_local(const BrickGeom &g, const int iv[3]) {
-  return (uint32_t)((iv[0] & (g.b[0] - 1)) * brick_pitch0(g) +
-                    (iv[1] & (g.b[1] - 1)) * brick_pitch1(g) + (iv[2] & (g.b[2] - 1)));
+  return (uint32_t)(__mul24(iv[0] & (g.b[0] - 1), brick_pitch0(g)) +
+                    __mul24(iv[1] & (g.b[1] - 1), brick_pitch1(g)) + (iv[2] & (g.b[2] - 1)));
+}
+
+// base[i] = v with the byte offset formed in 32 bits (the caller ensures
+// i * sizeof(V) < 2^32): the store takes the scalar-base + 32-bit vector-offset
+// form, with no 64-bit address arithmetic per store.
+template <typename V>
+__device__ __forceinline__ void store_at(V *base, uint32_t i, V v) {
+  *reinterpret_cast<V *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(V)) = v;
 }
 
 // A brick's key: its coordinates packed as (bx << ks0) | (by << ks1) | bz,
@@ -145,6 +154,11 @@ __device__ __forceinline__ uint32_t brick_key(const BrickGeom &g, const int iv[3
          (uint32_t)(iv[2] >> g.lg[2]);
 }
 constexpr uint32_t kNoBrick = 0xffffffffu;  // no key: keys stay under 2^31
+
+// Dynamic LDS of the count and fill passes: one word per brick key and bin.
+inline size_t hist_lds_bytes(const BrickGeom &g) {
+  return g.keyed ? 4 * (size_t)g.nkeys * (size_t)g.nt : 0;
+}
 
 // The brick index (axis 2 fastest, then axes 1 and 0, then bins) of a key of
 // bin `bin`, and the key of a brick index.
@@ -250,12 +264,14 @@ constexpr uint32_t kPairsOverflow = 0xffffffffu;
 constexpr size_t kWgPairBytes = 8 * (size_t)(kWgPairs + 1);
 
 // pairs: the workgroup's kWgPairBytes slot of per-workgroup pairs (or null);
-// per_ray: per-ray step counts out (or null).
+// per_ray: per-ray step counts out (or null).  Dynamic LDS: hist_lds_bytes(g),
+// one word per brick key (none without g.keyed), so that a geometry with few
+// bricks leaves the CU's LDS to more workgroups.
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom g,
                                                            uint32_t *counts, uint32_t *pairs,
                                                            uint32_t *per_ray) {
-  __shared__ uint32_t hist[kHistMax];
+  extern __shared__ uint32_t hist[];
   __shared__ uint32_t npairs;
   const bool lds = g.keyed;
   const int nkeys = g.nkeys * g.nt;
@@ -492,7 +508,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
                                                           const unsigned long long *offsets,
                                                           uint32_t *cursor, const uint32_t *mult,
                                                           RecPtrs<T> rec, const uint32_t *pairs) {
-  __shared__ uint32_t slot[kHistMax];
+  extern __shared__ uint32_t slot[];  // hist_lds_bytes(g)
   const bool lds = g.keyed;
   const bool direct = lds && g.direct;
   const int nkeys = g.nkeys * g.nt;
@@ -538,7 +554,12 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
     }
     unsigned long long p;
     if (direct) {
-      p = atomicAdd(&slot[key], 1u);
+      const uint32_t q = atomicAdd(&slot[key], 1u);
+      store_at(rec.rid, q, rid);
+      store_at(rec.d0, q, r.D[0]);
+      store_at(rec.d1, q, r.D[1]);
+      store_at(rec.d2, q, r.D[2]);
+      p = q;
     } else {
       const int b = brick_of_key(g, key, bin);
       const uint32_t s = lds ? atomicAdd(&slot[key], 1u) : atomicAdd(&cursor[b], 1u);
@@ -548,14 +569,17 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill(KArgs<T> a, BrickGeom 
         q = (q * mult[b]) % cnt;
       }
       p = offsets[b] + q;
+      rec.rid[p] = rid;
+      rec.d0[p] = r.D[0];
+      rec.d1[p] = r.D[1];
+      rec.d2[p] = r.D[2];
     }
-    rec.rid[p] = rid;
-    rec.d0[p] = r.D[0];
-    rec.d1[p] = r.D[1];
-    rec.d2[p] = r.D[2];
     pcur = p;
     lcur = brick_local(g, r.iv);
-  }, [&](uint32_t steps) { rec.lq[pcur] = lcur | (steps << 16); });
+  }, [&](uint32_t steps) {
+    if (direct) store_at(rec.lq, (uint32_t)pcur, lcur | (steps << 16));
+    else rec.lq[pcur] = lcur | (steps << 16);
+  });
 }
 
 // The brick is accumulated in double for both element types: on gfx950 the
@@ -1334,11 +1358,11 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
       return;
     }
     if (sun)
-      hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g, counts,
-                         pl, per_ray);
+      hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), hist_lds_bytes(g),
+                         s, av, g, counts, pl, per_ray);
     else
-      hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
-                         counts, pl, per_ray);
+      hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), hist_lds_bytes(g),
+                         s, av, g, counts, pl, per_ray);
   });
   // bit 0: each trace workgroup's per-brick counts (a canonical plan: the
   // count matrix and the tiles' mask lists) in scratch; bit 1: every ray's
@@ -1436,8 +1460,9 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   a.ray_steps = ((p->pairs_ready & 2) && p->ray_steps_out) ? p->ray_steps_out : nullptr;
   BrickGeom g;
   if ((rc = make_geom(c, p, g))) return rc;
-  // record positions fit the fill's 32-bit LDS cursors
-  g.direct = (!g.shuffle && (unsigned long long)p->records < (1ull << 32)) ? 1 : 0;
+  // record positions fit the fill's 32-bit LDS cursors, and byte offsets into
+  // the record arrays 32 bits
+  g.direct = (!g.shuffle && (unsigned long long)p->records * sizeof(T) < (1ull << 32)) ? 1 : 0;
   const MetaLayout m = meta_layout(g.nbricks);
   if (!p->meta || p->meta_bytes < m.total) return set_error(TG_EINVAL, "plan meta buffer too small");
   if (!p->rec || p->rec_bytes < rec_bytes<T>((size_t)p->records, call_rays(c)))
@@ -1479,11 +1504,11 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
   rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
     const uint32_t *pl = pairs ? pairs + (size_t)av.t0 * per_view * (kWgPairBytes / 4) : nullptr;
     if (sun)
-      hipLaunchKernelGGL((k_plan_fill<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
-                         offsets, cursor, mult, rp, pl);
+      hipLaunchKernelGGL((k_plan_fill<T, true>), dim3(blocks), dim3(kPlanBlock), hist_lds_bytes(g),
+                         s, av, g, offsets, cursor, mult, rp, pl);
     else
-      hipLaunchKernelGGL((k_plan_fill<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av, g,
-                         offsets, cursor, mult, rp, pl);
+      hipLaunchKernelGGL((k_plan_fill<T, false>), dim3(blocks), dim3(kPlanBlock), hist_lds_bytes(g),
+                         s, av, g, offsets, cursor, mult, rp, pl);
   });
   if (rc) return rc;
   if ((rc = hip_check(hipGetLastError(), "plan fill launch"))) return rc;

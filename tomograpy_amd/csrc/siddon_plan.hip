@@ -645,10 +645,12 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *wsum) 
 // Sort of the records of each pass (kBpjBlock consecutive records of a
 // chunk) by decreasing step count, ties by tie_pos of the pass position:
 // the ascending order of the keys ((0xffff - steps) << 10) | tie_pos(position).
-// A pass whose records all take fewer than 256 steps (every pass of a usual
-// brick) is sorted by one stable counting pass over the digit 255 - steps in
-// tie_pos order; any other pass by a bitonic network over the full keys.  The
-// two give the same order.
+// A pass whose records all take at most `top` steps (every pass: a ray visits
+// at most b0 + b1 + b2 - 2 voxels of a brick, its indices moving one way per
+// axis; top = b0 + b1 + b2 when that is under 256, else 255) is sorted by one
+// stable counting pass over the digit top - steps in tie_pos order, which
+// scans top + 2 buckets per wavefront; any other pass by a bitonic network
+// over the full keys.  The two give the same order.
 constexpr int kSortWaves = kBpjBlock / 64;
 constexpr int kSortBuckets = 257;  // digits 0..255, and 256 for lanes past the chunk
 
@@ -669,6 +671,11 @@ __global__ __launch_bounds__(kBpjBlock) __attribute__((amdgpu_waves_per_eu(8))) 
   chunk_range(g, offsets, chunk_prefix, blockIdx.x, r0, r1);
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
+  // the most steps a record takes in a brick, and the (digit, wavefront)
+  // counters in use: digits 0..top, then top + 1 for lanes past the chunk
+  const uint32_t edges = (uint32_t)(g.b[0] + g.b[1] + g.b[2]);
+  const uint32_t top = edges < 256u ? edges : 255u;
+  const int nhist = (int)(top + 2) * kSortWaves;
   // each lane's record of the next pass, loaded while this pass is sorted (the
   // passes are disjoint ranges, each rewritten in place)
   uint32_t n_rid = 0, n_lq = 0;
@@ -704,7 +711,7 @@ __global__ __launch_bounds__(kBpjBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     // apart in the pass rather than from adjacent pixels, which in a coarse
     // cube step through the same voxels and serialise on the same LDS cells
     const uint32_t tp = tie_pos(tid, g.spread);
-    if (__syncthreads_or(valid && steps > 255u)) {
+    if (__syncthreads_or(valid && steps > top)) {
       key[tid] = valid ? ((0xffffu - steps) << 10) | tp : 0xffffffffu;
       __syncthreads();
       for (uint32_t k = 2; k <= kBpjBlock; k <<= 1) {  // bitonic sort, ascending
@@ -725,8 +732,8 @@ __global__ __launch_bounds__(kBpjBlock) __attribute__((amdgpu_waves_per_eu(8))) 
       key[tid] = tie_pos(key[tid] & (kBpjBlock - 1), g.spread);
     } else {
       // lane tid's digit at its place tp in tie order
-      dig[tp] = (uint16_t)(valid ? 255u - steps : 256u);
-      for (int e = tid; e < kSortBuckets * kSortWaves; e += kBpjBlock) hist[e] = 0;
+      dig[tp] = (uint16_t)(valid ? top - steps : top + 1u);
+      for (int e = tid; e < nhist; e += kBpjBlock) hist[e] = 0;
       __syncthreads();
       // element v = tid of the tie order: its rank among the equal digits of
       // its wavefront (lanes agree on all 9 digit bits), and the wavefront's
@@ -742,12 +749,12 @@ __global__ __launch_bounds__(kBpjBlock) __attribute__((amdgpu_waves_per_eu(8))) 
       if (rank == 0) hist[d * kSortWaves + wave] = (uint32_t)__popcll(peers);
       __syncthreads();
       // exclusive scan of the counts in (digit, wavefront) order
-      constexpr int per = (kSortBuckets * kSortWaves + kBpjBlock - 1) / kBpjBlock;
+      const int per = (nhist + kBpjBlock - 1) / kBpjBlock;
       const int e0 = (int)tid * per;
       uint32_t local = 0;
-      for (int e = e0; e < e0 + per && e < kSortBuckets * kSortWaves; e++) local += hist[e];
+      for (int e = e0; e < e0 + per && e < nhist; e++) local += hist[e];
       uint32_t run = block_incl_scan(local, wsum) - local;
-      for (int e = e0; e < e0 + per && e < kSortBuckets * kSortWaves; e++) {
+      for (int e = e0; e < e0 + per && e < nhist; e++) {
         const uint32_t c = hist[e];
         hist[e] = run;
         run += c;

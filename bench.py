@@ -22,6 +22,8 @@ and the per-step access bytes of the survey's model (access_bytes):
   backward access = 2*S*b + R_t*b + V*18*b (voxel read-modify-write per step, pixel read per ray)
 which the brick backprojection serves from LDS and the forward's gathers from
 L2, so that access rate exceeds the HBM peak and is not a fraction of it.
+adjoint: <P x, y> against <x, P^T y> on the timed pair's own outputs (sums in
+float64), with the tolerance the survey's target sets (1e-5 in fp32).
 cpu_baseline: the oracle (C/OpenMP restatement of the reference) on a bounded
 sample of the same workload's views, on the host cores; "parity" compares the
 GPU pair with the oracle's output on that sample.
@@ -746,6 +748,12 @@ def main():
                           t_fwd if dom == "siddon_backproject" else t_bwd,
                           B_fwd if dom == "siddon_backproject" else B_bwd, args, plan_info)
 
+    # the adjoint identity <P x, y> = <x, P^T y> on the timed pair's outputs
+    lhs = float((P.xout.double() * y.double()).sum())
+    rhs = float((x.double() * P.xin.double()).sum())
+    adjoint = {"lhs": lhs, "rhs": rhs, "rel_diff": abs(lhs - rhs) / max(abs(lhs), abs(rhs)),
+               "tolerance": 1e-5 if dtype == np.float32 else 1e-12}
+
     # the pair from a cold operator: per-ray step counts, plan build, one pair
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -762,7 +770,7 @@ def main():
     del Pc
     torch.cuda.empty_cache()
 
-    extra = {}
+    extra = {"adjoint": adjoint}
     # the float32 results whose error against float64 the fp64 leg states
     ref32 = None
     if dtype == np.float32 and args.cg_iters:

@@ -155,7 +155,8 @@ def test_c2_operator_pair_on_16_views(oracle, dtype, tol):
     siddon_lo with its defaults -- precounted forward, brick-plan backward with
     the default 32 x 16 x 32 bricks (float64 sums in float32): forward bit-exact
     against the oracle, backward (rmatvec, lo_wrapper.py:29-34) within tol of the
-    oracle's single-thread backprojection (C:923-928)."""
+    oracle's single-thread backprojection (C:923-928), and the adjoint identity
+    <P x, y> = <x, P^T y> within tol."""
     from tomograpy_amd.lo_wrapper import siddon_lo
     print("C2 pair on 16 views", np.dtype(dtype).name, flush=True)
     ch, hs, rng = _c2_subset(dtype, C2_VIEWS)
@@ -176,6 +177,11 @@ def test_c2_operator_pair_on_16_views(oracle, dtype, tol):
     err = rel_l2(b, bref)
     print("C2 backward rel L2", err, "records", plan.records, "chunks", plan.chunks, flush=True)
     assert err <= tol
+    # the adjoint identity <P x, y> = <x, P^T y> on the same pair (sums in float64)
+    lhs = float(np.dot(y.astype(np.float64).ravel(), yy.astype(np.float64).ravel()))
+    rhs = float(np.dot(x.astype(np.float64).ravel(), b.astype(np.float64).ravel()))
+    print("C2 adjoint", lhs, rhs, flush=True)
+    assert abs(lhs - rhs) <= tol * max(abs(lhs), abs(rhs))
 
 
 def test_c2_cg_fp64_matches_scipy(oracle):

@@ -17,13 +17,13 @@ launch stream), against 614.4 G instructions/s (256 CUs x 2.4 GHz x one wave64
 VALU instruction per CU per clock = the 157.3 TFLOP/s packed-FP32 peak / 256
 flops).  Beside it: the HBM bytes per launch from the request-size-resolved
 TCC read/write counters of the same summary against 8 TB/s, the L2 hit rate,
-and the per-step access bytes of the survey's model (access_bytes):
+and the per-step access bytes of the algorithmic model (access_bytes):
   forward  access = S*b + R_t*b + V*18*b   (voxel gather per step, pixel write per ray)
   backward access = 2*S*b + R_t*b + V*18*b (voxel read-modify-write per step, pixel read per ray)
 which the brick backprojection serves from LDS and the forward's gathers from
 L2, so that access rate exceeds the HBM peak and is not a fraction of it.
 adjoint: <P x, y> against <x, P^T y> on the timed pair's own outputs (sums in
-float64), with the tolerance the survey's target sets (1e-5 in fp32).
+float64), against a tolerance of 1e-5 in fp32 (1e-12 in fp64).
 cpu_baseline: the oracle (C/OpenMP restatement of the reference) on a bounded
 sample of the same workload's views, on the host cores; "parity" compares the
 GPU pair with the oracle's output on that sample.

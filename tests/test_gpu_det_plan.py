@@ -139,3 +139,46 @@ def test_deterministic_long_buckets(oracle, dtype):
     oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref, y, nthread=1)
     got = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
     assert np.array_equal(got, ref)
+
+
+def test_canonical_fill_of_tiles_without_mask_lists(oracle):
+    """The count pass of a canonical plan hands each trace tile's lane masks to
+    the fill in a list of up to 255 bricks; a tile whose rays enter more bricks
+    is filled by tracing its masks first.  No tile of C1 enters that many, so
+    the test marks every other tile's list as overflowed between the count and
+    the build: the plan must still give the oracle's nthread = 1 sums bit for
+    bit."""
+    print("canonical fill without mask lists", flush=True)
+    ch, cube, hs = W.config1(np.float32)
+    P = siddon_lo(hs, ch, dtype=np.float32, deterministic=True)
+    lib = N.lib()
+    c = P._call_plain_bpj()
+    p = N.TgBpjPlan()
+    p.order = 3
+    N.check(lib.tg_bpj_plan_init(ctypes.byref(c), ctypes.byref(p)))
+    meta = torch.empty(int(p.meta_bytes), dtype=torch.uint8, device="cuda")
+    p.meta, p.meta_bytes = meta.data_ptr(), meta.numel()
+    scratch = torch.empty(int(p.scratch_bytes), dtype=torch.uint8, device="cuda")
+    p.scratch, p.scratch_bytes = scratch.data_ptr(), scratch.numel()
+    steps = torch.empty(P.nv * P.n1 * P.n2, dtype=torch.int32, device="cuda")
+    p.ray_steps_out = steps.data_ptr()
+    N.check(lib.tg_bpj_plan_count(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
+    torch.cuda.synchronize()
+    # the scratch ends with one list of 2304 words per trace tile (256 pixels
+    # of one image row); word 0 is the tile's brick count, all ones = overflow
+    tiles = P.nv * P.n1 * ((P.n2 + 255) // 256)
+    lists = scratch.view(torch.int32)[-2304 * tiles:].view(tiles, 2304)
+    assert bool((lists[:, 0] >= 0).all())
+    lists[::2, 0] = -1
+    rec = torch.empty(int(p.rec_bytes), dtype=torch.uint8, device="cuda")
+    p.rec, p.rec_bytes = rec.data_ptr(), rec.numel()
+    N.check(lib.tg_bpj_plan_build(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
+    torch.cuda.synchronize()
+    p.scratch, p.scratch_bytes, p.pairs_ready, p.ray_steps_out = None, 0, 0, None
+    P._det_plan, P._det_plan_bufs = p, (meta, rec)
+    y = np.random.default_rng(20).uniform(-1, 1, (128, 128, len(hs))).astype(np.float32)
+    ref = np.zeros(cube.shape, np.float32)
+    oracle.siddon(oracle.BPJ, view_records(hs), map_record(ch), ref, y, nthread=1)
+    out = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
+    T.check_status()
+    assert np.array_equal(out, ref)

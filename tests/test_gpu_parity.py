@@ -428,6 +428,27 @@ def test_plan_backprojection_edges(oracle, n, brick, chunk):
     assert rel_l2(out, ref) <= 1e-12
 
 
+@pytest.mark.parametrize("brick,order", [((2, 2, 2), 2), ((8, 8, 8), 1)])
+def test_plan_variants_global_counts_and_permuted_order(oracle, brick, order):
+    """2^3 bricks of a 40^3 cube have 20 480 brick keys, more than the count and
+    fill passes keep in LDS: the counts and cursors are global atomics.  Plan
+    order 1 places a brick's records through a permutation of its range, from
+    the brick offsets.  Both within 1e-12 of the oracle (fp64), with the Sun."""
+    from tomograpy_amd.lo_wrapper import siddon_lo
+    print("plan variants", brick, order, flush=True)
+    ch, x, data = C.small_case(n=40, npix=32, nv=6, dtype=np.float64, radius=9.0)
+    y = np.random.default_rng(27).normal(size=data.shape)
+    ref = np.zeros((40, 40, 40))
+    oracle.siddon(oracle.BPJ, view_records(data.header), map_record(ch), ref,
+                  T.InfoArray(y, header=data.header), sun=True, nthread=1)
+    P = siddon_lo(data.header, ch, dtype=np.float64, obstacle="sun", bpj_mode="plan", brick=brick,
+                  plan_order=order)
+    out = P.rmatvec(torch.from_numpy(y).cuda()).cpu().numpy()
+    T.check_status()
+    assert int(P.build_plan().order) == order
+    assert rel_l2(out, ref) <= 1e-12
+
+
 def test_plan_backprojection_time_bins(oracle):
     from tomograpy_amd.lo_wrapper import Siddon4d
     ch, x, data = C.small_case(n=20, npix=24, nv=8, dtype=np.float64, radius=9.0)

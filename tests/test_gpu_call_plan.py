@@ -2,7 +2,8 @@
 
 ``backprojector(data, cube)`` (siddon.py:119-151) adds P^T data into the cube.
 From ``siddon.PLAN_CALL_MIN_STEPS`` (a bound on the steps: rays x (n0 + n1 +
-n2)) it goes through a brick plan built for the call instead of one global
+n2)), or from ``siddon.PLAN_CALL_DENSE_MIN_STEPS`` with at least one ray per
+voxel, it goes through a brick plan built for the call instead of one global
 atomic per step.  Either way the result accumulates into the cube and agrees
 with the oracle's single-thread order within 1e-5 relative L2 (fp32) / 1e-12
 (fp64); the forward projection of the same geometry is unaffected.
@@ -97,15 +98,39 @@ def test_large_call_with_pixel_mask(oracle, spy):
     assert rel_l2(cube, ref) <= 1e-5
 
 
-def test_small_call_keeps_atomic_scatter(spy):
-    """Under the bound the call takes the atomic scatter (no plan is built)."""
-    ch, cube, hs = W.config1(np.float32)
-    assert 128 * 128 * len(hs) * 3 * 64 < S.PLAN_CALL_MIN_STEPS
-    y = np.random.default_rng(34).uniform(-1, 1, (128, 128, len(hs))).astype(np.float32)
-    out = T.InfoArray(np.zeros((64, 64, 64), np.float32), header=dict(ch))
+@pytest.mark.parametrize("n,nv,npx", [(48, 16, 96), (96, 32, 128)])
+def test_small_or_sparse_call_keeps_atomic_scatter(spy, n, nv, npx):
+    """Under S.PLAN_CALL_DENSE_MIN_STEPS (48^3 by 16 views of 96^2), or under
+    S.PLAN_CALL_MIN_STEPS with fewer rays than voxels (96^3 by 32 of 128^2), the
+    call takes the atomic scatter (no plan is built)."""
+    ch = W.cube_header(n, dtype=np.float32)
+    hs = W.view_headers(npx, 2 * np.pi * np.arange(nv) / nv, np.zeros(nv), dtype=np.float32)
+    bound = npx * npx * nv * 3 * n
+    assert bound < S.PLAN_CALL_MIN_STEPS
+    assert bound < S.PLAN_CALL_DENSE_MIN_STEPS or npx * npx * nv < n ** 3
+    y = np.random.default_rng(34).uniform(-1, 1, (npx, npx, nv)).astype(np.float32)
+    out = T.InfoArray(np.zeros((n, n, n), np.float32), header=dict(ch))
     T.backprojector(T.InfoArray(y, header=hs), out)
     assert spy == []
     assert np.abs(np.asarray(out)).sum() > 0
+
+
+def test_dense_call_through_plan_matches_oracle(oracle, spy):
+    """C1 (64^3 by 32 views of 128^2: 1.0e8 steps bound, two rays per voxel)
+    takes a plan under S.PLAN_CALL_MIN_STEPS, its rays colliding on voxels in
+    an atomic scatter; the sums agree with the oracle's single-thread order."""
+    print("call plan dense C1", flush=True)
+    ch, cube, hs = W.config1(np.float32)
+    bound = 128 * 128 * len(hs) * 3 * 64
+    assert S.PLAN_CALL_DENSE_MIN_STEPS <= bound < S.PLAN_CALL_MIN_STEPS
+    y = np.random.default_rng(36).uniform(-1, 1, (128, 128, len(hs))).astype(np.float32)
+    ref = np.zeros((64, 64, 64), np.float32)
+    oracle.siddon(oracle.BPJ, view_records(hs), map_record(ch), ref, T.InfoArray(y, header=hs),
+                  nthread=1)
+    out = T.InfoArray(np.zeros((64, 64, 64), np.float32), header=dict(ch))
+    T.backprojector(T.InfoArray(y, header=hs), out)
+    assert spy == [True]
+    assert rel_l2(out, ref) <= 1e-5
 
 
 @pytest.mark.parametrize("explicit_bins", [False, True])

@@ -249,13 +249,18 @@ def _mask_tensor(mask, shape, dev):
 # (n0 + n1 + n2)) goes through a brick plan built for the call -- count, fill,
 # then the brick kernel -- instead of one global atomic per step.  Measured
 # per call, build included, on device arrays: 256^3 seen by 128 views of 512^2
-# in 26.2 against 155 ms; 128^3 by 64 of 512^2 in 8.7 against 79 ms; 128^3 by
-# 64 of 256^2 in 2.7 against 10.9 ms.  Near 10^8 the two are alike (96^3 by 32
-# of 128^2: 0.92 against 0.73 ms).  A plan used once is left in trace order:
-# sorting its passes by step count (what the operators do for the many
-# backprojections of a solve) costs more than it saves in one (at 256^3,
-# 128 x 512^2: 35.5 ms sorted against 29.9 ms per call).
+# in 20.2 against 155 ms; 128^3 by 64 of 512^2 in 6.9 against 79 ms; 128^3 by
+# 64 of 256^2 in 2.0 against 10.7 ms.  Below it, a call of at least
+# PLAN_CALL_DENSE_MIN_STEPS with at least one ray per voxel takes a plan too:
+# there the atomics of neighbouring rays collide on the same voxels (64^3 by 32
+# of 128^2: 0.55 against 0.92 ms; 64^3 by 16 of 128^2: 0.45 against 0.50 ms),
+# while sparser or smaller calls keep the scatter (96^3 by 32 of 128^2: 0.63
+# against 0.59 ms; 48^3 by 16 of 96^2: 0.39 against 0.22 ms).  A plan used
+# once is left in trace order: sorting its passes by step count (what the
+# operators do for the many backprojections of a solve) costs more than it
+# saves in one (at 256^3, 128 x 512^2: 35.5 ms sorted against 29.9 ms).
 PLAN_CALL_MIN_STEPS = 1 << 28
+PLAN_CALL_DENSE_MIN_STEPS = 1 << 25
 
 
 def _plan_backprojection(data_header, cube_header, d, c, mask, obstacle, tdtype, dev,
@@ -323,8 +328,11 @@ def _siddon(kind, data, cube, mask=None, obstacle=None, t=None, ray=None, determ
     if deterministic and bwd:
         flags |= N.TG_DETERMINISTIC
     ws = None
+    rays, bound = n1 * n2 * nv, n1 * n2 * nv * sum(c.t.shape[:3])
+    # (voxels x time bins: the rays of a 4-D call spread over its bins)
+    dense = bound >= PLAN_CALL_DENSE_MIN_STEPS and rays >= int(np.prod(c.t.shape))
     planned = (bwd and not deterministic and t is None and ray is None
-               and n1 * n2 * nv * sum(c.t.shape[:3]) >= PLAN_CALL_MIN_STEPS
+               and (bound >= PLAN_CALL_MIN_STEPS or dense)
                and n1 * n2 * nv < (1 << 32) and max(c.t.shape[:3]) <= 1024
                and _plan_backprojection(data_header, cube_header, d, c, mask, obstacle, tdtype,
                                         dev, tindex))

@@ -421,12 +421,12 @@ __global__ __launch_bounds__(256) void k_det_colapply(uint32_t *__restrict__ mat
   }
 }
 
-// Fill of a canonical plan: each brick's lane mask (from the count pass's list
-// for the tile, or, when the list overflowed, from a first trace), then a
-// trace that writes each record at the brick's offset + the tile's scanned
-// count + the number of earlier lanes in the mask.  With g.direct the first
-// two are summed per brick into LDS before the trace (record positions are
-// under 2^32), so a record's position takes no global load.
+// Fill of a canonical plan, for the tiles whose count pass could not list
+// their masks: each brick's lane mask from a first trace, then a trace that
+// writes each record at the brick's offset + the tile's scanned count + the
+// number of earlier lanes in the mask.  With g.direct the first two are summed
+// per brick into LDS before the trace (record positions are under 2^32), so a
+// record's position takes no global load.
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, BrickGeom g,
                                                                 const unsigned long long *offsets,
@@ -436,33 +436,23 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, Bric
   __shared__ __align__(16) uint32_t mask[kDetMaxBricks * (kPlanBlock / 32)];
   __shared__ uint32_t base[kDetMaxBricks];
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  // tiles whose count pass listed their masks are filled by k_plan_fill_canon_slots
+  if (lists[(size_t)tile * kCanonListWords] != kPairsOverflow) return;
   const int words = g.nbricks * (kPlanBlock / 32);
   for (int k = threadIdx.x; k < words; k += blockDim.x) mask[k] = 0;
   __syncthreads();
   const uint32_t lane = threadIdx.x;
   const uint32_t bit = 1u << (lane & 31);
   const uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
-  const uint32_t *list = lists + (size_t)tile * kCanonListWords;
-  const uint32_t nl = list[0];
-  if (nl != kPairsOverflow) {
-    for (uint32_t e = threadIdx.x; e < 8 * nl; e += blockDim.x)
-      mask[list[1 + (e >> 3)] * 8 + (e & 7)] = list[1 + kCanonList + e];
-    if (g.direct)
-      for (uint32_t q = threadIdx.x; q < nl; q += blockDim.x) {
-        const uint32_t b = list[1 + q];
-        base[b] = (uint32_t)(offsets[b] + row[b]);
-      }
-  } else {
-    trace_bricks<T, SUN>(
-        a, g, tile,
-        [&](uint32_t key, int bin, const Ray<T> &, uint32_t, bool) {
-          atomicOr(&mask[brick_of_key(g, key, bin) * 8 + (lane >> 5)], bit);
-        },
-        [](uint32_t) {});
-    if (g.direct)
-      for (int b = threadIdx.x; b < g.nbricks; b += blockDim.x)
-        base[b] = (uint32_t)(offsets[b] + row[b]);
-  }
+  trace_bricks<T, SUN>(
+      a, g, tile,
+      [&](uint32_t key, int bin, const Ray<T> &, uint32_t, bool) {
+        atomicOr(&mask[brick_of_key(g, key, bin) * 8 + (lane >> 5)], bit);
+      },
+      [](uint32_t) {});
+  if (g.direct)
+    for (int b = threadIdx.x; b < g.nbricks; b += blockDim.x)
+      base[b] = (uint32_t)(offsets[b] + row[b]);
   __syncthreads();
   unsigned long long pcur = 0;
   uint32_t lcur = 0;
@@ -494,6 +484,76 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, Bric
     pcur = p;
     lcur = brick_local(g, r.iv);
   }, [&](uint32_t steps) { rec.lq[pcur] = lcur | (steps << 16); });
+}
+
+// Fill of a canonical plan for the tiles with a mask list: the listed bricks'
+// slots (brick -> list position, its mask and its first record position) in
+// LDS, about 18 KiB instead of a mask per brick of the map, then one trace
+// that writes each record at its brick's position + the number of earlier
+// lanes in the brick's mask.  Dynamic LDS: 4 bytes per brick.
+constexpr uint32_t kNoSlot = 0xffffffffu;
+constexpr unsigned long long kNoRecord = ~0ull;
+
+template <typename T, bool SUN>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon_slots(
+    KArgs<T> a, BrickGeom g, const unsigned long long *offsets, const uint32_t *rowmat,
+    const uint32_t *lists, RecPtrs<T> rec) {
+  extern __shared__ uint32_t slot[];
+  __shared__ __align__(16) uint32_t smask[kCanonList * 8];
+  __shared__ unsigned long long sbase[kCanonList];
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t *list = lists + (size_t)tile * kCanonListWords;
+  const uint32_t nl = list[0];
+  if (nl == kPairsOverflow) return;  // k_plan_fill_canon fills this tile
+  const uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
+  for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) slot[k] = kNoSlot;
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < 8 * nl; e += blockDim.x) smask[e] = list[1 + kCanonList + e];
+  for (uint32_t q = threadIdx.x; q < nl; q += blockDim.x) {
+    const uint32_t b = list[1 + q];
+    slot[b] = q;
+    sbase[q] = offsets[b] + row[b];
+  }
+  __syncthreads();
+  const uint32_t lane = threadIdx.x;
+  unsigned long long pcur = kNoRecord;
+  uint32_t lcur = 0;
+  trace_bricks<T, SUN>(a, g, tile, [&](uint32_t key, int bin, const Ray<T> &r, uint32_t rid,
+                                       bool first) {
+    if (first) {
+      RayEntry<T> e;
+      e.pabs[0] = r.pabs[0];
+      e.pabs[1] = r.pabs[1];
+      e.pabs[2] = r.pabs[2];
+      e.up = pack_up(r.up);
+      rec.ray[rid] = e;
+    }
+    const uint32_t q = slot[brick_of_key(g, key, bin)];
+    if (q >= nl) {
+      // a brick the count pass did not list: never for the same traversal;
+      // flagged, and nothing is written
+      flag_status(a.status, TG_EGEOMETRY);
+      pcur = kNoRecord;
+      return;
+    }
+    // the lanes before this one that enter the brick: whole 64-bit mask words
+    // of the earlier wavefronts (a count uniform per wavefront), then this
+    // wavefront's word below the lane
+    const unsigned long long *m = reinterpret_cast<const unsigned long long *>(&smask[q * 8]);
+    const uint32_t wv = lane >> 6;
+    uint32_t rank = 0;
+    for (uint32_t w = 0; w < wv; w++) rank += (uint32_t)__popcll(m[w]);
+    rank += (uint32_t)__popcll(m[wv] & ((1ull << (lane & 63)) - 1ull));
+    const unsigned long long p = sbase[q] + rank;
+    rec.rid[p] = rid;
+    rec.d0[p] = r.D[0];
+    rec.d1[p] = r.D[1];
+    rec.d2[p] = r.D[2];
+    pcur = p;
+    lcur = brick_local(g, r.iv);
+  }, [&](uint32_t steps) {
+    if (pcur != kNoRecord) rec.lq[pcur] = lcur | (steps << 16);
+  });
 }
 
 // pairs: this workgroup's slot of the count pass's pairs (or null): when it
@@ -1494,12 +1554,18 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
     rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
       const uint32_t *rm = rowmat + (size_t)av.t0 * per_view * (size_t)g.nbricks;
       const uint32_t *ls = lists + (size_t)av.t0 * per_view * kCanonListWords;
-      if (sun)
+      const size_t lds = (size_t)g.nbricks * 4;
+      if (sun) {
+        hipLaunchKernelGGL((k_plan_fill_canon_slots<T, true>), dim3(blocks), dim3(kPlanBlock), lds,
+                           s, av, g, offsets, rm, ls, rp);
         hipLaunchKernelGGL((k_plan_fill_canon<T, true>), dim3(blocks), dim3(kPlanBlock), 0, s, av,
                            g, offsets, rm, ls, rp);
-      else
+      } else {
+        hipLaunchKernelGGL((k_plan_fill_canon_slots<T, false>), dim3(blocks), dim3(kPlanBlock), lds,
+                           s, av, g, offsets, rm, ls, rp);
         hipLaunchKernelGGL((k_plan_fill_canon<T, false>), dim3(blocks), dim3(kPlanBlock), 0, s, av,
                            g, offsets, rm, ls, rp);
+      }
     });
     if (rc) return rc;
     return hip_check(hipGetLastError(), "canonical plan fill launch");

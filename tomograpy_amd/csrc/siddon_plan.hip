@@ -212,7 +212,7 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
   const uint32_t rid = ray_id((uint32_t)a.n1, (uint32_t)a.n2, t, i, j);
   const uint32_t kboff = (uint32_t)bin * (uint32_t)g.nkeys;
   uint32_t last = kNoBrick;
-  uint32_t seg = 0, total = 0;
+  uint32_t seg = 0;
   if (a.ray_steps) {
     // the ray's step count from the count pass, which walked this same
     // traversal with the map and obstacle tests: the walk needs neither
@@ -233,7 +233,8 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
     if (out) *out = n;
     return;
   }
-  int guard = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
+  const int guard0 = (int)(a.n[0] + a.n[1] + a.n[2]) + 2;
+  int guard = guard0;  // also counts the steps: guard0 - guard of them
   while (in_map(a, r) && (!SUN || outside_sun(h.M, u0, r.ac))) {
     if (--guard < 0) {
       flag_status(a.status, TG_EGEOMETRY);
@@ -249,10 +250,9 @@ __device__ __forceinline__ void trace_bricks(const KArgs<T> &a, const BrickGeom 
     bool sx, sy, sz;
     step_min(r, sx, sy, sz);
     seg++;
-    total++;
   }
   if (last != kNoBrick) close(seg);
-  if (out) *out = total;
+  if (out) *out = (uint32_t)(guard0 - (guard < 0 ? 0 : guard));
 }
 
 // Per-workgroup record counts the count pass hands to the fill pass, so that

@@ -141,14 +141,14 @@ def test_deterministic_long_buckets(oracle, dtype):
     assert np.array_equal(got, ref)
 
 
-def test_canonical_fill_of_tiles_without_mask_lists(oracle):
-    """The count pass of a canonical plan hands each trace tile's lane masks to
-    the fill in a list of up to 255 bricks; a tile whose rays enter more bricks
-    is filled by tracing its masks first.  No tile of C1 enters that many, so
-    the test marks every other tile's list as overflowed between the count and
-    the build: the plan must still give the oracle's nthread = 1 sums bit for
-    bit."""
-    print("canonical fill without mask lists", flush=True)
+def test_canonical_fill_of_tiles_without_brick_lists(oracle):
+    """The count pass of a canonical plan lists, per trace tile, up to 255
+    bricks its rays enter, and the fill keeps the lane masks of those bricks in
+    LDS slots; a tile whose rays enter more bricks is filled with a mask per
+    brick of the map instead.  No tile of C1 enters that many, so the test marks
+    every other tile's list as overflowed between the count and the build: the
+    plan must still give the oracle's nthread = 1 sums bit for bit."""
+    print("canonical fill without brick lists", flush=True)
     ch, cube, hs = W.config1(np.float32)
     P = siddon_lo(hs, ch, dtype=np.float32, deterministic=True)
     lib = N.lib()
@@ -164,10 +164,10 @@ def test_canonical_fill_of_tiles_without_mask_lists(oracle):
     p.ray_steps_out = steps.data_ptr()
     N.check(lib.tg_bpj_plan_count(ctypes.byref(c), ctypes.byref(p), S.stream_handle()))
     torch.cuda.synchronize()
-    # the scratch ends with one list of 2304 words per trace tile (256 pixels
-    # of one image row); word 0 is the tile's brick count, all ones = overflow
+    # the scratch ends with one list of 512 words per trace workgroup (256
+    # pixels of one image row); word 0 counts its bricks, all ones = overflow
     tiles = P.nv * P.n1 * ((P.n2 + 255) // 256)
-    lists = scratch.view(torch.int32)[-2304 * tiles:].view(tiles, 2304)
+    lists = scratch.view(torch.int32)[-512 * tiles:].view(tiles, 512)
     assert bool((lists[:, 0] >= 0).all())
     lists[::2, 0] = -1
     rec = torch.empty(int(p.rec_bytes), dtype=torch.uint8, device="cuda")

@@ -264,13 +264,15 @@ constexpr uint32_t kPairsOverflow = 0xffffffffu;
 constexpr size_t kWgPairBytes = 8 * (size_t)(kWgPairs + 1);
 
 // pairs: the workgroup's kWgPairBytes slot of per-workgroup pairs (or null);
-// per_ray: per-ray step counts out (or null).  Dynamic LDS: hist_lds_bytes(g),
-// one word per brick key (none without g.keyed), so that a geometry with few
-// bricks leaves the CU's LDS to more workgroups.
+// per_ray: per-ray step counts out (or null); rowmat (a canonical plan, which
+// needs g.keyed): this launch's rows of the tile x brick count matrix, the
+// workgroup writing its tile's.  Dynamic LDS: hist_lds_bytes(g), one word per
+// brick key (none without g.keyed), so that a geometry with few bricks leaves
+// the CU's LDS to more workgroups.
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom g,
                                                            uint32_t *counts, uint32_t *pairs,
-                                                           uint32_t *per_ray) {
+                                                           uint32_t *per_ray, uint32_t *rowmat) {
   extern __shared__ uint32_t hist[];
   __shared__ uint32_t npairs;
   const bool lds = g.keyed;
@@ -306,6 +308,19 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
     }
     __syncthreads();
     if (pairs && threadIdx.x == 0) pairs[0] = npairs <= (uint32_t)kWgPairs ? npairs : kPairsOverflow;
+    if (rowmat) {
+      // every brick's count, through its key (keys past a bin's bricks along
+      // axes 1 and 2 name no brick)
+      uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
+      for (int k = threadIdx.x; k < nkeys; k += blockDim.x) {
+        const int bin = g.nt > 1 ? k / g.nkeys : 0;
+        const uint32_t c = (uint32_t)(k - bin * g.nkeys);
+        if (((c >> g.ks[1]) & ((1u << (g.ks[0] - g.ks[1])) - 1u)) >= (uint32_t)g.nb[1] ||
+            (c & ((1u << g.ks[1]) - 1u)) >= (uint32_t)g.nb[2])
+          continue;
+        row[brick_of_key(g, (uint32_t)k, bin)] = hist[k];
+      }
+    }
   } else if (pairs && threadIdx.x == 0) {
     pairs[0] = kPairsOverflow;
   }
@@ -323,62 +338,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_count(KArgs<T> a, BrickGeom
 constexpr int kDetMaxBricks = 2048;  // 256-bit lane masks of all bricks in LDS (64 KiB)
 constexpr int kDetSegRows = 256;     // rows per segment of the column scan
 constexpr int kDetMinSlots = 256;    // fewest contribution slots of one deterministic batch
-// The count pass of a canonical plan builds every tile's lane masks as it
-// counts, and hands them to the fill in a list per tile: a header word (the
-// number of bricks the tile's rays enter, or kPairsOverflow), then up to
-// kCanonList brick indices and their 8 mask words each.  A tile of C2's
-// geometry enters up to about 180 bricks of 32 x 16 x 16.
-constexpr int kCanonList = 255;
-constexpr size_t kCanonListWords = 2304;  // 1 + 9 * kCanonList, rounded up to 256 bytes
-static_assert(kCanonListWords >= 1 + 9 * (size_t)kCanonList, "canonical mask list too short");
-
-// The canonical count: per-brick lane masks in LDS (g.nbricks x 8 words of
-// dynamic LDS), then the tile's row of the count matrix (popcounts), the
-// global per-brick counts and the tile's mask list.
-template <typename T, bool SUN>
-__global__ __launch_bounds__(kPlanBlock) void k_plan_count_canon(KArgs<T> a, BrickGeom g,
-                                                                 uint32_t *counts,
-                                                                 uint32_t *per_ray,
-                                                                 uint32_t *rowmat,
-                                                                 uint32_t *lists) {
-  extern __shared__ uint32_t cmask[];
-  __shared__ uint32_t nlist;
-  const int words = g.nbricks * (kPlanBlock / 32);
-  for (int k = threadIdx.x; k < words; k += blockDim.x) cmask[k] = 0;
-  if (threadIdx.x == 0) nlist = 0;
-  __syncthreads();
-  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t lane = threadIdx.x;
-  const uint32_t bit = 1u << (lane & 31);
-  trace_bricks<T, SUN>(
-      a, g, tile,
-      [&](uint32_t key, int bin, const Ray<T> &, uint32_t, bool) {
-        atomicOr(&cmask[brick_of_key(g, key, bin) * 8 + (lane >> 5)], bit);
-      },
-      [](uint32_t) {}, per_ray);
-  __syncthreads();
-  uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
-  uint32_t *list = lists + (size_t)tile * kCanonListWords;
-  for (int b = threadIdx.x; b < g.nbricks; b += blockDim.x) {
-    uint32_t m[8], cnt = 0;
-#pragma unroll
-    for (int w = 0; w < 8; w++) {
-      m[w] = cmask[b * 8 + w];
-      cnt += __popc(m[w]);
-    }
-    row[b] = cnt;
-    if (!cnt) continue;
-    atomicAdd(&counts[b], cnt);
-    const uint32_t q = atomicAdd(&nlist, 1u);
-    if (q < (uint32_t)kCanonList) {
-      list[1 + q] = (uint32_t)b;
-#pragma unroll
-      for (int w = 0; w < 8; w++) list[1 + kCanonList + 8 * q + w] = m[w];
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) list[0] = nlist <= (uint32_t)kCanonList ? nlist : kPairsOverflow;
-}
+// The canonical plan's count is the plan count with the tile's row of the
+// count matrix; its per-workgroup pairs (the bricks a tile's rays enter, up to
+// kWgPairs; a tile of C2's geometry enters up to about 180 bricks of
+// 32 x 16 x 16) name the bricks whose lane masks the fill builds.
 
 // Segment sums of the columns: block (s, cb) sums rows [s * 256, (s + 1) * 256)
 // of columns [cb * 256, cb * 256 + 256) into seg[s * nb + col].
@@ -422,7 +385,7 @@ __global__ __launch_bounds__(256) void k_det_colapply(uint32_t *__restrict__ mat
 }
 
 // Fill of a canonical plan, for the tiles whose count pass could not list
-// their masks: each brick's lane mask from a first trace, then a trace that
+// their bricks: each brick's lane mask from a first trace, then a trace that
 // writes each record at the brick's offset + the tile's scanned count + the
 // number of earlier lanes in the mask.  With g.direct the first two are summed
 // per brick into LDS before the trace (record positions are under 2^32), so a
@@ -431,13 +394,13 @@ template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, BrickGeom g,
                                                                 const unsigned long long *offsets,
                                                                 const uint32_t *rowmat,
-                                                                const uint32_t *lists,
+                                                                const uint32_t *pairs,
                                                                 RecPtrs<T> rec) {
   __shared__ __align__(16) uint32_t mask[kDetMaxBricks * (kPlanBlock / 32)];
   __shared__ uint32_t base[kDetMaxBricks];
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  // tiles whose count pass listed their masks are filled by k_plan_fill_canon_slots
-  if (lists[(size_t)tile * kCanonListWords] != kPairsOverflow) return;
+  // tiles whose count pass listed their bricks are filled by k_plan_fill_canon_slots
+  if (pairs[(size_t)blockIdx.x * (kWgPairBytes / 4)] != kPairsOverflow) return;
   const int words = g.nbricks * (kPlanBlock / 32);
   for (int k = threadIdx.x; k < words; k += blockDim.x) mask[k] = 0;
   __syncthreads();
@@ -486,36 +449,46 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon(KArgs<T> a, Bric
   }, [&](uint32_t steps) { rec.lq[pcur] = lcur | (steps << 16); });
 }
 
-// Fill of a canonical plan for the tiles with a mask list: the listed bricks'
-// slots (brick -> list position, its mask and its first record position) in
-// LDS, about 18 KiB instead of a mask per brick of the map, then one trace
-// that writes each record at its brick's position + the number of earlier
-// lanes in the brick's mask.  Dynamic LDS: 4 bytes per brick.
+// Fill of a canonical plan for the tiles whose count pass listed their bricks:
+// one LDS slot per listed brick (brick -> list position, its lane mask and its
+// first record position: about 18 KiB instead of a mask per brick of the
+// map), a first trace that sets each lane's bit in the masks of the bricks it
+// enters, then a trace that writes each record at its brick's position + the
+// number of earlier lanes in the brick's mask.  Dynamic LDS: 4 bytes per brick.
 constexpr uint32_t kNoSlot = 0xffffffffu;
 constexpr unsigned long long kNoRecord = ~0ull;
 
 template <typename T, bool SUN>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_fill_canon_slots(
     KArgs<T> a, BrickGeom g, const unsigned long long *offsets, const uint32_t *rowmat,
-    const uint32_t *lists, RecPtrs<T> rec) {
+    const uint32_t *pairs, RecPtrs<T> rec) {
   extern __shared__ uint32_t slot[];
-  __shared__ __align__(16) uint32_t smask[kCanonList * 8];
-  __shared__ unsigned long long sbase[kCanonList];
+  __shared__ __align__(16) uint32_t smask[kWgPairs * 8];
+  __shared__ unsigned long long sbase[kWgPairs];
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t *list = lists + (size_t)tile * kCanonListWords;
-  const uint32_t nl = list[0];
+  const uint32_t *pl = pairs + (size_t)blockIdx.x * (kWgPairBytes / 4);
+  const uint32_t nl = pl[0];
   if (nl == kPairsOverflow) return;  // k_plan_fill_canon fills this tile
   const uint32_t *row = rowmat + (size_t)tile * (size_t)g.nbricks;
   for (int k = threadIdx.x; k < g.nbricks; k += blockDim.x) slot[k] = kNoSlot;
+  for (uint32_t e = threadIdx.x; e < 8 * nl; e += blockDim.x) smask[e] = 0;
   __syncthreads();
-  for (uint32_t e = threadIdx.x; e < 8 * nl; e += blockDim.x) smask[e] = list[1 + kCanonList + e];
   for (uint32_t q = threadIdx.x; q < nl; q += blockDim.x) {
-    const uint32_t b = list[1 + q];
+    const uint32_t b = pl[2 + 2 * q];
     slot[b] = q;
     sbase[q] = offsets[b] + row[b];
   }
   __syncthreads();
   const uint32_t lane = threadIdx.x;
+  const uint32_t bit = 1u << (lane & 31);
+  trace_bricks<T, SUN>(
+      a, g, tile,
+      [&](uint32_t key, int bin, const Ray<T> &, uint32_t, bool) {
+        const uint32_t q = slot[brick_of_key(g, key, bin)];
+        if (q < nl) atomicOr(&smask[q * 8 + (lane >> 5)], bit);
+      },
+      [](uint32_t) {});
+  __syncthreads();
   unsigned long long pcur = kNoRecord;
   uint32_t lcur = 0;
   trace_bricks<T, SUN>(a, g, tile, [&](uint32_t key, int bin, const Ray<T> &r, uint32_t rid,
@@ -1323,9 +1296,9 @@ MetaLayout meta_layout(int nbricks) {
 // Scratch of a canonical plan: the tile x brick count matrix and the segment
 // sums of its column scan.
 // Scratch of a canonical plan's build: the count matrix (tiles x bricks),
-// its column segment sums and every tile's mask list.
+// its column segment sums and every tile's pairs (the bricks it enters).
 struct CanonScratch {
-  size_t mat, seg, lists, total;
+  size_t mat, seg, pairs, total;
   int64_t rows, nseg;
 };
 
@@ -1335,8 +1308,8 @@ CanonScratch canon_scratch(int64_t tiles, int nbricks) {
   c.nseg = (tiles + kDetSegRows - 1) / kDetSegRows;
   c.mat = 0;
   c.seg = align_up(4 * (size_t)tiles * (size_t)nbricks, 256);
-  c.lists = align_up(c.seg + 4 * (size_t)c.nseg * (size_t)nbricks, 256);
-  c.total = align_up(c.lists + 4 * kCanonListWords * (size_t)tiles, 256);
+  c.pairs = align_up(c.seg + 4 * (size_t)c.nseg * (size_t)nbricks, 256);
+  c.total = align_up(c.pairs + kWgPairBytes * (size_t)tiles, 256);
   return c;
 }
 
@@ -1404,36 +1377,24 @@ int plan_count(const tg_call *c, tg_bpj_plan *p, hipStream_t s) {
   const CanonScratch cs = canon_scratch((int64_t)c->nv * (int64_t)per_view, g.nbricks);
   if (g.canon && (!p->scratch || p->scratch_bytes < cs.total))
     return set_error(TG_EINVAL, "a canonical plan needs a scratch of %zu bytes", cs.total);
-  uint32_t *pairs = (!g.canon && p->scratch && p->scratch_bytes >= plan_scratch_bytes(a))
-                        ? (uint32_t *)p->scratch
-                        : nullptr;
+  uint32_t *pairs = g.canon ? (uint32_t *)((char *)p->scratch + cs.pairs)
+                 : (p->scratch && p->scratch_bytes >= plan_scratch_bytes(a)) ? (uint32_t *)p->scratch
+                                                                             : nullptr;
   uint32_t *rowmat = g.canon ? (uint32_t *)((char *)p->scratch + cs.mat) : nullptr;
-  uint32_t *lists = g.canon ? (uint32_t *)((char *)p->scratch + cs.lists) : nullptr;
   uint32_t *per_ray = p->ray_steps_out;
   rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
     uint32_t *pl = pairs ? pairs + (size_t)av.t0 * per_view * (kWgPairBytes / 4) : nullptr;
-    if (g.canon) {
-      uint32_t *rm = rowmat + (size_t)av.t0 * per_view * (size_t)g.nbricks;
-      uint32_t *ls = lists + (size_t)av.t0 * per_view * kCanonListWords;
-      const size_t lds = (size_t)g.nbricks * (kPlanBlock / 32) * 4;
-      if (sun)
-        hipLaunchKernelGGL((k_plan_count_canon<T, true>), dim3(blocks), dim3(kPlanBlock), lds, s,
-                           av, g, counts, per_ray, rm, ls);
-      else
-        hipLaunchKernelGGL((k_plan_count_canon<T, false>), dim3(blocks), dim3(kPlanBlock), lds, s,
-                           av, g, counts, per_ray, rm, ls);
-      return;
-    }
+    uint32_t *rm = rowmat ? rowmat + (size_t)av.t0 * per_view * (size_t)g.nbricks : nullptr;
     if (sun)
       hipLaunchKernelGGL((k_plan_count<T, true>), dim3(blocks), dim3(kPlanBlock), hist_lds_bytes(g),
-                         s, av, g, counts, pl, per_ray);
+                         s, av, g, counts, pl, per_ray, rm);
     else
       hipLaunchKernelGGL((k_plan_count<T, false>), dim3(blocks), dim3(kPlanBlock), hist_lds_bytes(g),
-                         s, av, g, counts, pl, per_ray);
+                         s, av, g, counts, pl, per_ray, rm);
   });
-  // bit 0: each trace workgroup's per-brick counts (a canonical plan: the
-  // count matrix and the tiles' mask lists) in scratch; bit 1: every ray's
-  // step count in ray_steps_out
+  // bit 0: each trace workgroup's per-brick counts (a canonical plan: also
+  // the count matrix) in scratch; bit 1: every ray's step count in
+  // ray_steps_out
   p->pairs_ready = ((pairs || rowmat) ? 1 : 0) | (per_ray ? 2 : 0);
   if (rc) return rc;
   if ((rc = hip_check(hipGetLastError(), "plan count launch"))) return rc;
@@ -1550,10 +1511,10 @@ int plan_build(const tg_call *c, const tg_bpj_plan *p, hipStream_t s) {
     if (!(p->pairs_ready & 1) || !p->scratch || p->scratch_bytes < cs.total)
       return set_error(TG_EINVAL, "a canonical plan's build needs the scratch its count filled");
     const uint32_t *rowmat = (const uint32_t *)((const char *)p->scratch + cs.mat);
-    const uint32_t *lists = (const uint32_t *)((const char *)p->scratch + cs.lists);
+    const uint32_t *cpairs = (const uint32_t *)((const char *)p->scratch + cs.pairs);
     rc = for_view_launches(a, [&](const KArgs<T> &av, uint32_t blocks) {
       const uint32_t *rm = rowmat + (size_t)av.t0 * per_view * (size_t)g.nbricks;
-      const uint32_t *ls = lists + (size_t)av.t0 * per_view * kCanonListWords;
+      const uint32_t *ls = cpairs + (size_t)av.t0 * per_view * (kWgPairBytes / 4);
       const size_t lds = (size_t)g.nbricks * 4;
       if (sun) {
         hipLaunchKernelGGL((k_plan_fill_canon_slots<T, true>), dim3(blocks), dim3(kPlanBlock), lds,

@@ -9,14 +9,14 @@ cube buffer, backproject the images (siddon_lo matvec + rmatvec,
 lo_wrapper.py:23-34).  Inputs are resident in HBM before timing starts.
 
 value = Siddon steps per second (voxel segments traversed by the pair / time).
-roofline: the dominant kernel against the resource that binds it, instruction
-issue: its wave64 VALU instructions per launch (SQ_INSTS_VALU, from the
-committed counter summary profiles/siddon_c2_pair_counters.json when this run
-is the workload it measured) over its average launch time (HIP events on the
-launch stream), against 614.4 G instructions/s (256 CUs x 2.4 GHz x one wave64
-VALU instruction per CU per clock = the 157.3 TFLOP/s packed-FP32 peak / 256
-flops).  Beside it: the HBM bytes per launch from the request-size-resolved
-TCC read/write counters of the same summary against 8 TB/s, the L2 hit rate,
+roofline: the dominant kernel against the HBM roofline: its DRAM bytes per
+launch from the request-size-resolved TCC read/write counters of the committed
+summary profiles/siddon_c2_pair_counters.json (when this run is the workload it
+measured) over its average launch time (HIP events on the launch stream),
+against 8 TB/s.  Beside it: its wave64 VALU instructions per launch
+(SQ_INSTS_VALU) over the same time against 1228.8 G instructions/s (256 CUs x
+2.4 GHz x 2: four SIMDs of 32 lanes per CU for 32-bit operations, the
+157.3 TFLOP/s FP32 vector peak over 128 flops per wave64 FMA), the L2 hit rate,
 and the per-step access bytes of the algorithmic model (access_bytes):
   forward  access = S*b + R_t*b + V*18*b   (voxel gather per step, pixel write per ray)
   backward access = 2*S*b + R_t*b + V*18*b (voxel read-modify-write per step, pixel read per ray)
@@ -46,8 +46,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# wave64 VALU instructions per second: 256 CUs x 2.4 GHz x 1 per CU per clock
-VALU_PEAK_GIPS = 256 * 2.4
+# wave64 VALU instructions per second: the 157.3 TFLOP/s FP32 vector peak over
+# 128 flops per wave64 FMA instruction = 256 CUs x 2.4 GHz x 2 per CU per clock
+# (4 SIMDs, each 32 lanes wide for 32-bit operations)
+VALU_PEAK_GIPS = 256 * 2.4 * 2
 COUNTERS = os.path.join(ROOT, "profiles", "siddon_c2_pair_counters.json")
 
 
@@ -533,28 +535,32 @@ def measured_counters(kernel, args, plan_info):
 
 def roofline(kernel, t_launch, access_bytes, args, plan_info):
     """The roofline object of the bench line for `kernel` at t_launch seconds
-    per launch (see the module docstring)."""
+    per launch (see the module docstring): HBM bytes per launch from the
+    committed counter summary against 8 TB/s, with VALU issue beside it."""
     c = measured_counters(kernel, args, plan_info)
-    out = {"bound": "valu", "kernel": kernel, "unit": "G VALU instructions/s",
-           "peak": VALU_PEAK_GIPS, "achieved": None, "frac": None, "traffic": None,
+    out = {"bound": "hbm", "kernel": kernel, "unit": "GB/s", "peak": HBM_PEAK_GBS,
+           "achieved": None, "frac": None, "traffic": None,
            "access_bytes": access_bytes, "access_GBs": access_bytes / t_launch / 1e9}
     if c is None:
         out["note"] = "no counter summary for this workload (profiles/ covers the default)"
         return out
     valu = c["valu_instructions"]
     hbm = c["hbm_read_bytes"] + c["hbm_write_bytes"]
-    out.update({"achieved": valu / t_launch / 1e9,
-                "frac": valu / t_launch / 1e9 / VALU_PEAK_GIPS,
+    out.update({"achieved": hbm / t_launch / 1e9,
+                "frac": hbm / t_launch / 1e9 / HBM_PEAK_GBS,
                 "traffic": hbm,
-                "valu_instructions_per_launch": valu,
-                "hbm": {"achieved_GBs": hbm / t_launch / 1e9, "peak_GBs": HBM_PEAK_GBS,
-                        "frac": hbm / t_launch / 1e9 / HBM_PEAK_GBS,
-                        "read_bytes": c["hbm_read_bytes"], "write_bytes": c["hbm_write_bytes"]},
+                "read_bytes": c["hbm_read_bytes"], "write_bytes": c["hbm_write_bytes"],
+                "valu": {"instructions_per_launch": valu,
+                         "achieved_GIPS": valu / t_launch / 1e9, "peak_GIPS": VALU_PEAK_GIPS,
+                         "frac": valu / t_launch / 1e9 / VALU_PEAK_GIPS},
                 "l2_hit_rate": c["l2_hit_rate"],
                 "counters": os.path.relpath(COUNTERS, ROOT)})
-    for k in ("lds_instructions", "l2_atomics", "issue_frac_at_measured_clock", "clock_GHz"):
+    for k in ("lds_instructions", "l2_atomics", "clock_GHz", "wait_inst_lds_frac"):
         if k in c:
             out[k] = c[k]
+    if "clock_GHz" in c:
+        out["valu"]["frac_at_measured_clock"] = (valu / t_launch / 1e9
+                                                 / (VALU_PEAK_GIPS * c["clock_GHz"] / 2.4))
     return out
 
 
